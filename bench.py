@@ -1,0 +1,185 @@
+"""Benchmark: forward frames/s of the splat renderer on one MI355X.
+
+Workload (one step = one forward frame, inputs resident in HBM): 1,000,000
+clustered splats (64 clusters, centres U[-3,3]^3, sigma 0.3, log-scales
+N(-5, 0.7), SH degree 3), camera at (0,0,-8) looking at the origin, 50 deg
+vertical FOV, 1920x1080, 16x16 tiles, seed 1.  Synthetic data.
+
+Prints one JSON line.  `roofline` describes the stage that takes the largest
+share of the frame, timed with HIP events on the renderer's stream over the
+timed region; `cpu_baseline` is the oracle's FP32 CPU pipeline (a port of the
+reference WGSL pass graph, OpenMP) timed on this host on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "web-gs-renderer_amd"))
+
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(stage: str, n_splats: int, instances: int, visible: int, width: int, height: int,
+                      tiles: int, key_bytes: int, sort_passes: int) -> float:
+    """Algorithmic HBM bytes per launch of each stage."""
+    if stage == "preprocess":
+        # read 236 B of parameters for a visible splat (56 B when culled before SH),
+        # write tiles 4 + meta 16 + conic 16 + xy 8 + rgb 16
+        return visible * 236 + (n_splats - visible) * 56 + n_splats * 60
+    if stage == "offset scan":
+        return n_splats * 8.0
+    if stage == "emit":
+        return n_splats * 20.0 + instances * (key_bytes + 4)
+    if stage == "sort":
+        # one histogram read of the keys, then per pass read + write of key and value
+        return instances * (key_bytes + sort_passes * 2 * (key_bytes + 4))
+    if stage == "tile ranges":
+        return instances * key_bytes + tiles * 8.0
+    if stage == "raster":
+        return instances * 44.0 + tiles * 8.0 + width * height * 12.0
+    return 0.0
+
+
+def cpu_baseline(sc, u, budget_s: float = 15.0):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    threads = min(os.cpu_count() or 1, 16)
+    oracle.lib().orc_set_threads(threads)
+    frames = 0
+    t0 = time.perf_counter()
+    packed = sc.packed()
+    while True:
+        oracle.render(packed, sc.count, u, use_radix=True)
+        frames += 1
+        if time.perf_counter() - t0 > budget_s or frames >= 8:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": frames / dt, "unit": "frames/s", "cores": threads, "kind": "port",
+            "sample": f"{frames} full frames of the same 1M-splat 1080p workload through the oracle's FP32 "
+                      f"C++ pass graph (OpenMP, {threads} threads, stable LSD radix sort)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the renderer has no "
+              f"multi-GPU path in this build.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from gsr import lib, scenes
+
+    if not torch.cuda.is_available():
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    log("building the 1M-splat scene")
+    sc, cam = scenes.workload_1m_1080p()
+    u = cam.uniform(sc.count)
+    ctx = lib.Context(0)
+    ctx.scene_upload("bench", sc.packed(), sc.count)
+    params, n = ctx.scene_params("bench")
+    opts = lib.options(tile_size=16, timing=False)
+    log(f"warmup {args.warmup} frames")
+    for _ in range(args.warmup):
+        ctx.forward(params, n, u, opts, None)
+    ctx.synchronize()
+    instances = ctx.instance_count()
+
+    # timed region: K forward frames, HIP events around every stage on the renderer's stream
+    topts = lib.options(tile_size=16, timing=True)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    log(f"timing {args.steps} frames")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.forward(params, n, u, topts, None)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    stage_ms = ctx.timings()
+    ms_per_step = dt * 1000.0 / args.steps
+    fps = 1000.0 / ms_per_step
+
+    W, H, tile = cam.width, cam.height, 16
+    tiles = ((W + tile - 1) // tile) * ((H + tile - 1) // tile)
+    tbits = max(1, (tiles - 1).bit_length())
+    key_bytes = 4 if 16 + tbits <= 32 else 8
+    passes = (16 + tbits + 7) // 8
+    # visible splats (non-zero tile count) from the device outputs of the last frame
+    visible = None
+    try:
+        import ctypes
+        from gsr.lib import _ptr  # noqa: F401
+        pre_tiles = torch.empty(n, dtype=torch.int32, device="cuda")
+        meta = torch.empty(4 * n, dtype=torch.int32, device="cuda")
+        conic = torch.empty(4 * n, dtype=torch.float32, device="cuda")
+        xy = torch.empty(2 * n, dtype=torch.float32, device="cuda")
+        rgb = torch.empty(4 * n, dtype=torch.float32, device="cuda")
+        ctx.preprocess(params, n, u, tile, pre_tiles, meta, conic, xy, rgb)
+        visible = int((pre_tiles > 0).sum().item())
+        del pre_tiles, meta, conic, xy, rgb
+    except Exception as e:  # pragma: no cover
+        log(f"visible count unavailable: {e}")
+        visible = n
+    stages = {k: v for k, v in stage_ms.items() if k != "TOTAL"}
+    dominant = max(stages, key=stages.get)
+    dom_ms = stages[dominant]
+    dom_bytes = algorithmic_bytes(dominant, n, instances, visible, W, H, tiles, key_bytes, passes)
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": dominant,
+                "kernel_ms": round(dom_ms, 5), "algorithmic_bytes": int(dom_bytes)}
+    sort_ms = stages.get("sort", 0.0)
+    extra = {
+        "stage_ms": {k: round(v, 5) for k, v in stage_ms.items()},
+        "instances": instances,
+        "visible_splats": visible,
+        "sort_gkeys_per_s": round(instances / (sort_ms * 1e-3) / 1e9, 4) if sort_ms else None,
+        "sort_gbs": round(algorithmic_bytes("sort", n, instances, visible, W, H, tiles, key_bytes, passes)
+                          / (sort_ms * 1e-3) / 1e9, 2) if sort_ms else None,
+    }
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline")
+        cpu = cpu_baseline(sc, u)
+    ctx.close()
+    out = {
+        "metric": "fwd & fwd+bwd frames/s @1M splats 1080p; 64-view batch vs plain; sort Gkeys/s",
+        "value": round(fps, 3),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "plain forward, 1,000,000 clustered splats, 1920x1080, SH degree 3, 16x16 tiles",
+                   "splats": n, "width": W, "height": H, "tile": tile, "key_bytes": key_bytes},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        **extra,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

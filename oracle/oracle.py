@@ -1,0 +1,149 @@
+"""ctypes access to the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg as the checker.  The renderer never imports it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "build", "liboracle.so")
+_lib = None
+
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+_u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
+
+
+def build():
+    import subprocess
+    subprocess.run(["make", "-C", _HERE, "-B"], check=True, stdout=subprocess.DEVNULL)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run make -C oracle)")
+        L = ctypes.CDLL(LIB_PATH)
+        L.orc_exp_f32.restype = ctypes.c_float
+        L.orc_exp_f32.argtypes = [ctypes.c_float]
+        L.orc_exp_f32_array.argtypes = [_f32p, _f32p, ctypes.c_int64]
+        L.orc_num_threads.restype = ctypes.c_int
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_preprocess_f32.argtypes = [_f32p, ctypes.c_uint32, _f32p, ctypes.c_uint32,
+                                         _u32p, _u32p, _f32p, _f32p, _f32p]
+        L.orc_preprocess_f64.argtypes = [_f32p, ctypes.c_uint32, _f32p, ctypes.c_uint32,
+                                         _u32p, _u32p, _f64p, _f64p, _f64p]
+        L.orc_scan.restype = ctypes.c_uint64
+        L.orc_scan.argtypes = [_u32p, ctypes.c_uint32, _u32p]
+        L.orc_emit.argtypes = [_u32p, _u32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                               ctypes.c_uint64, _u64p, _u32p]
+        L.orc_sort_stable.argtypes = [_u64p, _u32p, ctypes.c_uint64]
+        L.orc_sort_radix.argtypes = [_u64p, _u32p, ctypes.c_uint64, ctypes.c_uint32]
+        L.orc_tile_ranges.argtypes = [_u64p, ctypes.c_uint64, ctypes.c_uint32, _u32p]
+        L.orc_raster_f32.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p,
+                                     _f32p, _f32p, _f32p, _f32p, _f32p, _u32p]
+        L.orc_raster_f64.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p,
+                                     _f64p, _f64p, _f64p, _f64p, _f64p, _u32p]
+        L.orc_render_f32.restype = ctypes.c_int
+        L.orc_render_f32.argtypes = [_f32p, ctypes.c_uint32, _f32p, ctypes.c_uint32, ctypes.c_uint64,
+                                     _f32p, _f32p, _u32p, _u64p, _f64p, ctypes.c_int]
+        L.orc_render_f64.restype = ctypes.c_int
+        L.orc_render_f64.argtypes = [_f32p, ctypes.c_uint32, _f32p, ctypes.c_uint32, ctypes.c_uint64,
+                                     _f64p, _f64p, _u32p, _u64p]
+        _lib = L
+    return _lib
+
+
+def exp_f32(x):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.empty_like(x)
+    lib().orc_exp_f32_array(x.ravel(), y.ravel(), x.size)
+    return y
+
+
+def preprocess(packed, n, uni, tile=16, fp64=False):
+    dt = np.float64 if fp64 else np.float32
+    tiles = np.zeros(n, np.uint32)
+    meta = np.zeros(4 * n, np.uint32)
+    conic = np.zeros(4 * n, dt)
+    xy = np.zeros(2 * n, dt)
+    rgb = np.zeros(4 * n, dt)
+    f = lib().orc_preprocess_f64 if fp64 else lib().orc_preprocess_f32
+    f(np.ascontiguousarray(packed, np.float32), n, np.ascontiguousarray(uni, np.float32), tile,
+      tiles, meta, conic, xy, rgb)
+    return dict(tiles=tiles, meta=meta.reshape(n, 4), conic=conic.reshape(n, 4),
+                xy=xy.reshape(n, 2), rgb=rgb.reshape(n, 4))
+
+
+def scan(counts):
+    counts = np.ascontiguousarray(counts, np.uint32)
+    out = np.zeros_like(counts)
+    total = lib().orc_scan(counts, counts.size, out)
+    return out, int(total)
+
+
+def emit(offsets, meta, n, width, tile, cap):
+    offsets = np.ascontiguousarray(offsets, np.uint32)
+    m = min(int(offsets[n]) if offsets.size > n else 0, cap)
+    keys = np.zeros(max(m, 1), np.uint64)
+    vals = np.zeros(max(m, 1), np.uint32)
+    lib().orc_emit(offsets, np.ascontiguousarray(meta, np.uint32).ravel(), n, width, tile, cap, keys, vals)
+    return keys[:m], vals[:m]
+
+
+def sort_stable(keys, vals):
+    k = np.ascontiguousarray(keys, np.uint64).copy()
+    v = np.ascontiguousarray(vals, np.uint32).copy()
+    lib().orc_sort_stable(k, v, k.size)
+    return k, v
+
+
+def sort_radix(keys, vals, end_bit):
+    k = np.ascontiguousarray(keys, np.uint64).copy()
+    v = np.ascontiguousarray(vals, np.uint32).copy()
+    lib().orc_sort_radix(k, v, k.size, end_bit)
+    return k, v
+
+
+def tile_ranges(keys, num_tiles):
+    k = np.ascontiguousarray(keys, np.uint64)
+    r = np.zeros(2 * num_tiles, np.uint32)
+    lib().orc_tile_ranges(k, k.size, num_tiles, r)
+    return r
+
+
+def raster(width, height, tile, ranges, vals, conic, xy, rgb, fp64=False):
+    dt = np.float64 if fp64 else np.float32
+    out = np.zeros(3 * width * height, dt)
+    tf = np.zeros(width * height, dt)
+    last = np.zeros(width * height, np.uint32)
+    f = lib().orc_raster_f64 if fp64 else lib().orc_raster_f32
+    f(width, height, tile, np.ascontiguousarray(ranges, np.uint32), np.ascontiguousarray(vals, np.uint32),
+      np.ascontiguousarray(conic, dt).ravel(), np.ascontiguousarray(xy, dt).ravel(),
+      np.ascontiguousarray(rgb, dt).ravel(), out, tf, last)
+    return out.reshape(3, height, width), tf.reshape(height, width), last.reshape(height, width)
+
+
+def render(packed, n, uni, tile=16, cap=0, fp64=False, use_radix=False):
+    """Whole-frame oracle.  Returns (rgb[3,H,W], T_final[H,W], last[H,W], instances, stage_ms)."""
+    W, H = int(uni[40]), int(uni[41])
+    dt = np.float64 if fp64 else np.float32
+    out = np.zeros(3 * W * H, dt)
+    tf = np.zeros(W * H, dt)
+    last = np.zeros(W * H, np.uint32)
+    inst = np.zeros(1, np.uint64)
+    ms = np.zeros(6, np.float64)
+    uni = np.ascontiguousarray(uni, np.float32)
+    packed = np.ascontiguousarray(packed, np.float32)
+    if fp64:
+        lib().orc_render_f64(packed, n, uni, tile, cap, out, tf, last, inst)
+    else:
+        lib().orc_render_f32(packed, n, uni, tile, cap, out, tf, last, inst, ms, 1 if use_radix else 0)
+    return (out.reshape(3, H, W), tf.reshape(H, W), last.reshape(H, W), int(inst[0]), ms)

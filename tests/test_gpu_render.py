@@ -1,0 +1,126 @@
+"""Whole-frame forward through the C ABI against the whole-frame oracle.
+
+The frame (preprocess -> scan -> emit -> sort -> ranges -> raster on the GPU)
+must reproduce the oracle's FP32 image: within the stated 1e-4 max-abs and, by
+construction, bit for bit; the instance count must match exactly.  The FP64
+oracle's error is reported."""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import empty_dev, to_dev, to_host
+from gsr import lib, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def test_render_small_matches_oracle(torch_cuda, ctx, orc):
+    sc, cam = scenes.workload_small()
+    u = cam.uniform(sc.count)
+    ctx.scene_upload("small", sc.packed(), sc.count)
+    ctx.scene_select("small")
+    img = ctx.render(u, lib.options(timing=True))
+    ref, Tref, lref, I, _ = orc.render(sc.packed(), sc.count, u)
+    ref64, _, _, _, _ = orc.render(sc.packed(), sc.count, u, fp64=True)
+    assert ctx.instance_count() == I == 84916
+    assert np.abs(img - ref).max() <= 1e-4
+    assert np.array_equal(img, ref)
+    err64 = np.abs(img.astype(np.float64) - ref64)
+    _say(f"small: FP64 oracle error max {err64.max():.3e} mean {err64.mean():.3e}")
+    rows = ctx.timings()
+    assert {"preprocess", "offset scan", "emit", "sort", "tile ranges", "raster", "TOTAL"} <= set(rows)
+    assert rows["TOTAL"] > 0
+
+
+def test_forward_from_device_params(torch_cuda, ctx, orc):
+    sc, cam = scenes.workload_small()
+    cam.width, cam.height = 333, 217
+    u = cam.uniform(sc.count)
+    params = to_dev(torch_cuda, sc.planar())
+    out = empty_dev(torch_cuda, 3 * 333 * 217, np.float32)
+    torch_cuda.cuda.synchronize()
+    ctx.forward(params, sc.count, u, lib.options(save_state=True), out)
+    ctx.synchronize()
+    ref, _, _, I, _ = orc.render(sc.packed(), sc.count, u)
+    assert np.array_equal(to_host(out, np.float32).reshape(3, 217, 333), ref)
+
+
+@pytest.mark.parametrize("tile", [8, 32])
+def test_render_other_tile_sizes(torch_cuda, ctx, orc, tile):
+    sc, cam = scenes.workload_small()
+    u = cam.uniform(sc.count)
+    ctx.scene_upload("small", sc.packed(), sc.count)
+    ctx.scene_select("small")
+    img = ctx.render(u, lib.options(tile_size=tile))
+    ref, _, _, I, _ = orc.render(sc.packed(), sc.count, u, tile=tile)
+    assert np.array_equal(img, ref)
+
+
+def test_render_64bit_keys_equal_32bit(torch_cuda, ctx):
+    sc, cam = scenes.workload_small()
+    u = cam.uniform(sc.count)
+    ctx.scene_upload("small", sc.packed(), sc.count)
+    a = ctx.render(u, lib.options(key_bytes=lib.KEY_32))
+    b = ctx.render(u, lib.options(key_bytes=lib.KEY_64))
+    assert np.array_equal(a, b)
+
+
+def test_render_1m_1080p_matches_oracle(torch_cuda, ctx, orc):
+    sc, cam = scenes.workload_1m_1080p()
+    u = cam.uniform(sc.count)
+    _say("render 1M splats 1920x1080")
+    ctx.scene_upload("big", sc.packed(), sc.count)
+    ctx.scene_select("big")
+    img = ctx.render(u)
+    I = ctx.instance_count()
+    ref, _, _, Iref, _ = orc.render(sc.packed(), sc.count, u, use_radix=True)
+    _say(f"1M: instances gpu {I} oracle {Iref}; max abs {np.abs(img - ref).max():.3e}")
+    assert I == Iref
+    assert np.abs(img - ref).max() <= 1e-4
+    assert np.array_equal(img, ref)
+    ctx.scene_free("big")
+
+
+def test_capacity_overflow_is_reported(torch_cuda, ctx):
+    sc, cam = scenes.workload_small()
+    ctx.scene_upload("small", sc.packed(), sc.count)
+    ctx.scene_select("small")
+    with pytest.raises(lib.GsrError) as e:
+        ctx.render(cam.uniform(sc.count), lib.options(capacity=1000))
+    assert e.value.code == lib.GSR_ERR_CAPACITY
+    assert "84916" in str(e.value)
+    img = ctx.render(cam.uniform(sc.count))     # the context keeps working
+    assert np.isfinite(img).all()
+
+
+def test_scene_lifecycle(torch_cuda, ctx):
+    sc, cam = scenes.workload_small()
+    ctx.scene_upload("a", sc.packed(), sc.count)
+    ctx.scene_upload("b", sc.packed(), sc.count)
+    ctx.scene_select("b")
+    with pytest.raises(lib.GsrError) as e:
+        ctx.scene_select("nope")
+    assert e.value.code == lib.GSR_ERR_NOT_FOUND
+    ctx.scene_free("a")
+    with pytest.raises(lib.GsrError):
+        ctx.scene_free("a")
+    p, n = ctx.scene_params("b")
+    assert p and n == sc.count
+
+
+def test_rgba8_conversion(torch_cuda, ctx):
+    W, H = 40, 30
+    rgb = np.random.default_rng(0).uniform(-0.5, 1.5, size=(3, H, W)).astype(np.float32)
+    d = to_dev(torch_cuda, rgb.ravel())
+    torch_cuda.cuda.synchronize()
+    out = ctx.to_rgba8(d, W, H, flip_y=False)
+    ref = np.rint(np.clip(rgb, 0, 1) * 255).astype(np.uint8).transpose(1, 2, 0)
+    assert np.array_equal(out[..., :3], ref) and (out[..., 3] == 255).all()
+    flipped = ctx.to_rgba8(d, W, H, flip_y=True)
+    assert np.array_equal(flipped, out[::-1])

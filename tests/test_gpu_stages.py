@@ -1,0 +1,226 @@
+"""Stage-wise parity of the HIP kernels against the oracle, through the C ABI.
+
+Each stage receives the oracle's inputs, so integer outputs (counts, rects,
+depth keys, offsets, keys/values, ranges) must be bit-exact, and the FP32
+outputs must be bit-identical as well: kernels and oracle execute the same
+FP32 operation sequence (tolerance kept at the stated 1e-4 for colour)."""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import empty_dev, to_dev, to_host
+from gsr import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _pre_gpu(torch, ctx, sc, u, tile=16):
+    n = sc.count
+    params = to_dev(torch, sc.planar())
+    tiles = empty_dev(torch, n, np.uint32)
+    meta = empty_dev(torch, 4 * n, np.uint32)
+    conic = empty_dev(torch, 4 * n, np.float32)
+    xy = empty_dev(torch, 2 * n, np.float32)
+    rgb = empty_dev(torch, 4 * n, np.float32)
+    torch.cuda.synchronize()
+    ctx.preprocess(params, n, u, tile, tiles, meta, conic, xy, rgb)
+    return dict(tiles=to_host(tiles, np.uint32)[:n], meta=to_host(meta, np.uint32)[:4 * n].reshape(n, 4),
+                conic=to_host(conic, np.float32)[:4 * n].reshape(n, 4), xy=to_host(xy, np.float32)[:2 * n].reshape(n, 2),
+                rgb=to_host(rgb, np.float32)[:4 * n].reshape(n, 4))
+
+
+def _edge_scene(n=4096, seed=11):
+    """Uniform splats plus planted edge cases: near-plane, behind the camera,
+    off-screen, huge and tiny scales, unnormalised and degenerate quaternions."""
+    sc = scenes.uniform_scene(n, seed=seed, extent=2.5, scale_mu=-3.0, scale_sigma=1.5)
+    sc.pos[0] = (0.0, 0.0, -3.85)
+    sc.pos[1] = (0.0, 0.0, -5.0)
+    sc.pos[2] = (40.0, 0.0, 0.0)
+    sc.lscale[3] = (1.0, 1.0, 1.0)
+    sc.lscale[4] = (-12.0, -12.0, -12.0)
+    sc.rot[5] = (3.0, 0.0, 0.0, 0.0)
+    sc.rot[6] = (0.0, 0.0, 0.0, 1e-3)
+    sc.opacity[7] = 30.0
+    sc.opacity[8] = -30.0
+    return sc
+
+
+@pytest.mark.parametrize("which,tile", [("small", 16), ("edge", 16), ("edge", 8), ("edge", 32)])
+def test_preprocess_bit_exact(torch_cuda, ctx, orc, which, tile):
+    if which == "small":
+        sc, cam = scenes.workload_small()
+    else:
+        sc = _edge_scene()
+        cam = scenes.Camera((0.0, 0.0, -4.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 60.0, 300, 200)
+    u = cam.uniform(sc.count)
+    _say(f"preprocess {which} tile={tile} n={sc.count}")
+    g = _pre_gpu(torch_cuda, ctx, sc, u, tile)
+    o = orc.preprocess(sc.packed(), sc.count, u, tile=tile)
+    assert np.array_equal(g["tiles"], o["tiles"])
+    assert np.array_equal(g["meta"], o["meta"])
+    for k in ("conic", "xy", "rgb"):
+        assert np.array_equal(g[k], o[k]), (k, np.abs(g[k] - o[k]).max())
+
+
+@pytest.mark.parametrize("n", [0, 1, 100, 4095, 4096, 4097, 1_000_000, 3_000_001])
+def test_scan_exact(torch_cuda, ctx, orc, n):
+    _say(f"scan n={n}")
+    rng = np.random.default_rng(n)
+    counts = rng.integers(0, 64, size=n).astype(np.uint32)
+    d_in = to_dev(torch_cuda, counts if n else np.zeros(1, np.uint32))
+    d_out = empty_dev(torch_cuda, n + 1, np.uint32)
+    d_tot = empty_dev(torch_cuda, 1, np.uint32)
+    torch_cuda.cuda.synchronize()
+    ctx.scan_offsets(d_in, n, d_out, d_tot)
+    out = to_host(d_out, np.uint32)[:n + 1]
+    ref, total = orc.scan(np.concatenate([counts, [0]]).astype(np.uint32))
+    assert np.array_equal(out, ref)
+    assert int(to_host(d_tot, np.uint32)[0]) == total
+
+
+def test_scan_in_place(torch_cuda, ctx, orc):
+    counts = np.random.default_rng(1).integers(0, 9, size=50001).astype(np.uint32)
+    buf = to_dev(torch_cuda, np.concatenate([counts, [0]]).astype(np.uint32))
+    torch_cuda.cuda.synchronize()
+    ctx.scan_offsets(buf, counts.size, buf)
+    ref, _ = orc.scan(np.concatenate([counts, [0]]).astype(np.uint32))
+    assert np.array_equal(to_host(buf, np.uint32), ref)
+
+
+def _oracle_chain(orc, sc, u, tile=16):
+    n = sc.count
+    W = int(u[40])
+    pre = orc.preprocess(sc.packed(), n, u, tile=tile)
+    offs, total = orc.scan(np.concatenate([pre["tiles"], [0]]).astype(np.uint32))
+    keys, vals = orc.emit(offs, pre["meta"], n, W, tile, 64 * n)
+    return pre, offs, total, keys, vals
+
+
+@pytest.mark.parametrize("key_bytes", [4, 8])
+def test_emit_exact(torch_cuda, ctx, orc, key_bytes):
+    sc, cam = scenes.workload_small()
+    u = cam.uniform(sc.count)
+    pre, offs, total, keys, vals = _oracle_chain(orc, sc, u)
+    _say(f"emit key_bytes={key_bytes} instances={total}")
+    kdt = np.uint32 if key_bytes == 4 else np.uint64
+    d_off = to_dev(torch_cuda, offs)
+    d_meta = to_dev(torch_cuda, pre["meta"].ravel())
+    d_k = empty_dev(torch_cuda, total, kdt)
+    d_v = empty_dev(torch_cuda, total, np.uint32)
+    torch_cuda.cuda.synchronize()
+    ctx.emit(d_off, d_meta, sc.count, 512, 16, 64 * sc.count, key_bytes, d_k, d_v)
+    assert np.array_equal(to_host(d_k, kdt)[:total].astype(np.uint64), keys)
+    assert np.array_equal(to_host(d_v, np.uint32)[:total], vals)
+    # capacity truncation keeps the first slots only (emit.wgsl:57-59)
+    cap = total // 3
+    d_k2 = empty_dev(torch_cuda, total, kdt)
+    torch_cuda.cuda.synchronize()
+    ctx.emit(d_off, d_meta, sc.count, 512, 16, cap, key_bytes, d_k2, d_v)
+    got = to_host(d_k2, kdt).astype(np.uint64)
+    assert np.array_equal(got[:cap], keys[:cap]) and not got[cap:].any()
+
+
+def _gpu_sort(torch, ctx, keys, vals, key_bytes, begin, end):
+    n = keys.size
+    kdt = np.uint32 if key_bytes == 4 else np.uint64
+    k0 = to_dev(torch, keys.astype(kdt) if n else np.zeros(1, kdt))
+    v0 = to_dev(torch, vals if n else np.zeros(1, np.uint32))
+    k1 = empty_dev(torch, n, kdt)
+    v1 = empty_dev(torch, n, np.uint32)
+    torch.cuda.synchronize()
+    idx = ctx.sort_pairs(key_bytes, k0, v0, k1, v1, n, begin, end)
+    ks = (k1 if idx else k0)
+    vs = (v1 if idx else v0)
+    return to_host(ks, kdt)[:n].astype(np.uint64), to_host(vs, np.uint32)[:n]
+
+
+@pytest.mark.parametrize("n,key_bytes,bits", [
+    (0, 4, 32), (1, 4, 32), (100, 4, 32), (4095, 4, 29), (4096, 4, 29), (4097, 8, 45),
+    (100_000, 4, 32), (1_000_003, 4, 29), (3_000_000, 8, 45), (2_000_000, 8, 64)])
+def test_sort_stable_exact(torch_cuda, ctx, n, key_bytes, bits):
+    _say(f"sort n={n} key_bytes={key_bytes} bits={bits}")
+    rng = np.random.default_rng(n + bits)
+    hi = (1 << bits) - 1 if bits < 64 else np.iinfo(np.uint64).max
+    keys = rng.integers(0, hi, size=n, dtype=np.uint64, endpoint=True)
+    # heavy ties: half the keys drawn from a few values
+    if n > 10:
+        keys[: n // 2] = keys[rng.integers(0, 8, n // 2)]
+    vals = rng.permutation(n).astype(np.uint32)
+    k, v = _gpu_sort(torch_cuda, ctx, keys, vals, key_bytes, 0, bits)
+    order = np.argsort(keys, kind="stable")
+    assert np.array_equal(k, keys[order])
+    assert np.array_equal(v, vals[order])
+
+
+def test_sort_bit_window_and_all_equal(torch_cuda, ctx):
+    rng = np.random.default_rng(3)
+    keys = rng.integers(0, 1 << 32, size=300_000, dtype=np.uint64)
+    vals = np.arange(keys.size, dtype=np.uint32)
+    # bits [8, 24) only: stable on that window
+    k, v = _gpu_sort(torch_cuda, ctx, keys, vals, 4, 8, 24)
+    win = (keys >> np.uint64(8)) & np.uint64(0xFFFF)
+    order = np.argsort(win, kind="stable")
+    assert np.array_equal(v, vals[order])
+    same = np.full(70_000, 12345, np.uint64)
+    k, v = _gpu_sort(torch_cuda, ctx, same, np.arange(same.size, dtype=np.uint32), 4, 0, 32)
+    assert np.array_equal(v, np.arange(same.size, dtype=np.uint32))
+
+
+def test_sort_emitted_instances_exact(torch_cuda, ctx, orc):
+    sc, cam = scenes.workload_small()
+    u = cam.uniform(sc.count)
+    pre, offs, total, keys, vals = _oracle_chain(orc, sc, u)
+    ks, vs = orc.sort_stable(keys, vals)
+    for kb in (4, 8):
+        k, v = _gpu_sort(torch_cuda, ctx, keys, vals, kb, 0, 26)
+        assert np.array_equal(k, ks) and np.array_equal(v, vs)
+
+
+@pytest.mark.parametrize("key_bytes", [4, 8])
+def test_tile_ranges_exact(torch_cuda, ctx, orc, key_bytes):
+    sc, cam = scenes.workload_small()
+    u = cam.uniform(sc.count)
+    pre, offs, total, keys, vals = _oracle_chain(orc, sc, u)
+    ks, vs = orc.sort_stable(keys, vals)
+    kdt = np.uint32 if key_bytes == 4 else np.uint64
+    ref = orc.tile_ranges(ks, 32 * 32)
+    d_k = to_dev(torch_cuda, ks.astype(kdt))
+    d_r = to_dev(torch_cuda, np.full(2 * 1024, 7, np.uint32))  # stale contents must be cleared
+    torch_cuda.cuda.synchronize()
+    ctx.tile_ranges(d_k, key_bytes, ks.size, 1024, d_r)
+    assert np.array_equal(to_host(d_r, np.uint32), ref)
+    ctx.tile_ranges(d_k, key_bytes, 0, 1024, d_r)
+    assert not to_host(d_r, np.uint32).any()
+
+
+@pytest.mark.parametrize("tile", [8, 16, 32])
+def test_raster_bit_exact(torch_cuda, ctx, orc, tile):
+    sc, cam = scenes.workload_small()
+    cam.width, cam.height = 500, 380      # partial edge tiles
+    u = cam.uniform(sc.count)
+    W, H = 500, 380
+    pre, offs, total, keys, vals = _oracle_chain(orc, sc, u, tile)
+    ks, vs = orc.sort_stable(keys, vals)
+    TX, TY = (W + tile - 1) // tile, (H + tile - 1) // tile
+    ranges = orc.tile_ranges(ks, TX * TY)
+    _say(f"raster tile={tile} instances={total}")
+    img_o, T_o, last_o = orc.raster(W, H, tile, ranges, vs, pre["conic"], pre["xy"], pre["rgb"])
+    d = {k: to_dev(torch_cuda, np.ascontiguousarray(pre[k]).ravel()) for k in ("conic", "xy", "rgb")}
+    d_r = to_dev(torch_cuda, ranges)
+    d_v = to_dev(torch_cuda, vs)
+    out = empty_dev(torch_cuda, 3 * W * H, np.float32)
+    T = empty_dev(torch_cuda, W * H, np.float32)
+    last = empty_dev(torch_cuda, W * H, np.uint32)
+    torch_cuda.cuda.synchronize()
+    ctx.rasterize(W, H, tile, sc.count, vs.size, d_r, d_v, d["conic"], d["xy"], d["rgb"], out, T, last)
+    img = to_host(out, np.float32).reshape(3, H, W)
+    assert np.abs(img - img_o).max() <= 1e-4
+    assert np.array_equal(img, img_o)
+    assert np.array_equal(to_host(T, np.float32).reshape(H, W), T_o)
+    assert np.array_equal(to_host(last, np.uint32).reshape(H, W), last_o)

@@ -1,0 +1,213 @@
+"""Pins of the CPU oracle (no GPU): every oracle function is checked against a
+closed form, brute force, a special case or its FP64 twin, chosen so that a
+dropped term, a wrong sign or a transposed operand fails one of them."""
+import math
+
+import numpy as np
+import pytest
+
+from gsr import scenes
+
+
+def _identity_splat_scene(n=1, pos=(0.0, 0.0, 0.0), lscale=-2.0, opacity=0.0, dc=(0.0, 0.0, 0.0), rest=None):
+    s = scenes.Scene(pos=np.array([pos] * n, np.float32), lscale=np.full((n, 3), lscale, np.float32),
+                     rot=np.tile(np.array([1, 0, 0, 0], np.float32), (n, 1)),
+                     dc=np.array([dc] * n, np.float32), opacity=np.full(n, opacity, np.float32),
+                     rest=np.zeros((n, 45), np.float32) if rest is None else rest.astype(np.float32))
+    return s
+
+
+def test_exp_matches_libm(orc):
+    x = np.linspace(-87.0, 88.7, 400001, dtype=np.float32)
+    y = orc.exp_f32(x).astype(np.float64)
+    ref = np.exp(x.astype(np.float64))
+    ulp = np.spacing(ref.astype(np.float32)).astype(np.float64)
+    assert (np.abs(y - ref) / ulp).max() < 2.0
+    assert orc.exp_f32(np.array([0.0], np.float32))[0] == 1.0
+    assert orc.exp_f32(np.array([-100.0], np.float32))[0] == 0.0
+    assert np.isinf(orc.exp_f32(np.array([89.0], np.float32))[0])
+
+
+@pytest.mark.parametrize("n", [0, 1, 7, 4096, 100003])
+def test_scan_is_exclusive_prefix_sum(orc, n):
+    rng = np.random.default_rng(n)
+    counts = rng.integers(0, 300, size=n + 1).astype(np.uint32)
+    counts[n] = 0
+    out, total = orc.scan(counts)
+    ref = np.concatenate([[0], np.cumsum(counts.astype(np.uint64))[:-1]])
+    assert np.array_equal(out.astype(np.uint64), ref)
+    assert total == int(counts.astype(np.uint64).sum())
+
+
+def test_emit_matches_brute_force(orc):
+    rng = np.random.default_rng(5)
+    n, W, tile = 300, 200, 16
+    TX = (W + tile - 1) // tile
+    x0 = rng.integers(0, TX, n); x1 = np.minimum(x0 + rng.integers(0, 4, n), TX - 1)
+    y0 = rng.integers(0, 9, n); y1 = np.minimum(y0 + rng.integers(0, 4, n), 8)
+    d16 = rng.integers(0, 65536, n)
+    vis = rng.random(n) < 0.8
+    meta = np.stack([(y0 << 16) | x0, (y1 << 16) | x1, d16, np.zeros(n, int)], 1).astype(np.uint32)
+    counts = np.where(vis, (x1 - x0 + 1) * (y1 - y0 + 1), 0).astype(np.uint32)
+    offs, total = orc.scan(np.concatenate([counts, [0]]).astype(np.uint32))
+    keys, vals = orc.emit(offs, meta, n, W, tile, 1 << 30)
+    ek, ev = [], []
+    for g in range(n):
+        if not vis[g]:
+            continue
+        for ty in range(y0[g], y1[g] + 1):
+            for tx in range(x0[g], x1[g] + 1):
+                ek.append(((ty * TX + tx) << 16) | d16[g]); ev.append(g)
+    assert np.array_equal(keys, np.array(ek, np.uint64)) and np.array_equal(vals, np.array(ev, np.uint32))
+    # capacity drops the tail slots (emit.wgsl:57-59)
+    k2, v2 = orc.emit(offs, meta, n, W, tile, 100)
+    assert k2.size == 100 and np.array_equal(k2, keys[:100])
+
+
+@pytest.mark.parametrize("n,bits", [(0, 29), (1, 29), (1000, 8), (50000, 29), (200000, 45)])
+def test_sort_is_stable_lexicographic(orc, n, bits):
+    rng = np.random.default_rng(bits + n)
+    keys = rng.integers(0, 1 << min(bits, 20), size=n).astype(np.uint64) << np.uint64(max(0, bits - 20))
+    vals = np.arange(n, dtype=np.uint32)
+    rng.shuffle(vals)
+    k, v = orc.sort_stable(keys, vals)
+    order = np.lexsort((np.arange(n), keys))      # stable by key, ties by input position
+    assert np.array_equal(k, keys[order]) and np.array_equal(v, vals[order])
+    k2, v2 = orc.sort_radix(keys, vals, bits)
+    assert np.array_equal(k2, k) and np.array_equal(v2, v)
+
+
+def test_tile_ranges_closed_form(orc):
+    rng = np.random.default_rng(9)
+    tiles = np.sort(rng.integers(0, 50, 2000)).astype(np.uint64)
+    keys = (tiles << np.uint64(16)) | rng.integers(0, 65536, 2000).astype(np.uint64)
+    keys = np.sort(keys)
+    r = orc.tile_ranges(keys, 64).reshape(64, 2)
+    t = (keys >> np.uint64(16)).astype(np.int64)
+    for tt in range(64):
+        idx = np.nonzero(t == tt)[0]
+        if idx.size:
+            assert r[tt, 0] == idx[0] and r[tt, 1] == idx[-1] + 1
+        else:
+            assert r[tt, 0] == 0 and r[tt, 1] == 0
+    assert orc.tile_ranges(keys[:0], 64).sum() == 0
+
+
+def _cam(W=64, H=64, eye=(0.0, 0.0, -4.0), fov=60.0):
+    return scenes.Camera(eye, (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), fov, W, H)
+
+
+def test_preprocess_isotropic_on_axis_closed_form(orc):
+    """Identity rotation, isotropic scale s, splat at the origin on the optical
+    axis: Sigma = s^2 I, J diagonal (fx/z), conic = 1 / (s^2 f^2 / z^2 + 0.3)."""
+    s = 0.05
+    sc = _identity_splat_scene(lscale=math.log(s), opacity=0.0)
+    cam = _cam(256, 256)
+    u = cam.uniform(1)
+    for fp64 in (False, True):
+        o = orc.preprocess(sc.packed(), 1, u, fp64=fp64)
+        f = float(u[36]); z = 4.0
+        var = s * s * f * f / (z * z) + 0.3
+        tol = 1e-4 if not fp64 else 1e-6
+        assert abs(o["conic"][0, 0] - 1.0 / var) < tol * (1.0 / var)
+        assert abs(o["conic"][0, 1]) < 1e-6
+        assert abs(o["conic"][0, 2] - 1.0 / var) < tol * (1.0 / var)
+        assert o["conic"][0, 3] == 0.5                       # sigmoid(0)
+        assert abs(o["xy"][0, 0] - 127.5) < 1e-3 and abs(o["xy"][0, 1] - 127.5) < 1e-3
+        r = math.ceil(3.0 * math.sqrt(var))
+        x0 = int(math.floor((127.5 - r) / 16)); x1 = int(math.floor((127.5 + r) / 16))
+        assert o["meta"][0, 0] == (x0 << 16) | x0 and o["meta"][0, 1] == (x1 << 16) | x1
+        assert o["tiles"][0] == (x1 - x0 + 1) ** 2
+
+
+def test_preprocess_sh_degree0_and_clamp(orc):
+    """Zero higher bands: colour = C0 * dc + 0.5, clamped below at 0 (no upper clamp)."""
+    dc = (1.0, -3.0, 5.0)
+    sc = _identity_splat_scene(dc=dc)
+    o = orc.preprocess(sc.packed(), 1, _cam().uniform(1))
+    c0 = 0.28209479177387814
+    exp = [max(c0 * d + 0.5, 0.0) for d in dc]
+    assert np.allclose(o["rgb"][0, :3], exp, rtol=1e-6, atol=1e-7)
+    assert o["rgb"][0, 1] == 0.0 and o["rgb"][0, 2] > 1.0
+
+
+def test_preprocess_sh_band1_sign(orc):
+    """Band-1 term -C1*y*f0 + C1*z*f1 - C1*x*f2 with d = normalize(p - eye): a splat on
+    the +z axis seen from -z has d = (0,0,1), so only f1 contributes (+C1*f1)."""
+    rest = np.zeros((1, 45), np.float32)
+    rest[0, 0] = 1.0; rest[0, 1] = 2.0; rest[0, 2] = 4.0
+    sc = _identity_splat_scene(pos=(0.0, 0.0, 0.5), rest=rest)
+    o = orc.preprocess(sc.packed(), 1, _cam().uniform(1), fp64=True)
+    assert abs(o["rgb"][0, 0] - (0.4886025119029199 * 2.0 + 0.5)) < 1e-6
+    assert abs(o["rgb"][0, 1] - 0.5) < 1e-7   # other channels have zero coefficients
+
+
+def test_preprocess_near_and_frustum_cull(orc):
+    sc = _identity_splat_scene(n=3)
+    sc.pos[0] = (0.0, 0.0, -3.9)     # 0.1 in front of the eye: near-culled
+    sc.pos[1] = (50.0, 0.0, 0.0)     # far off to the side: frustum-culled
+    sc.pos[2] = (0.0, 0.0, 0.0)
+    o = orc.preprocess(sc.packed(), 3, _cam().uniform(3))
+    assert o["tiles"][0] == 0 and not o["conic"][0].any() and not o["xy"][0].any()
+    assert o["tiles"][1] == 0 and o["conic"][1].any() and not o["rgb"][1].any()
+    assert o["tiles"][2] > 0
+
+
+def test_preprocess_fp32_vs_fp64(orc):
+    sc, cam = scenes.workload_small()
+    u = cam.uniform(sc.count)
+    a = orc.preprocess(sc.packed(), sc.count, u)
+    b = orc.preprocess(sc.packed(), sc.count, u, fp64=True)
+    vis = a["tiles"] > 0
+    assert np.abs(a["xy"] - b["xy"]).max() < 2e-3
+    rel = np.abs(a["conic"] - b["conic"]) / (np.abs(b["conic"]) + 1e-3)
+    assert rel.max() < 1e-3
+    assert np.abs(a["rgb"][vis] - b["rgb"][vis]).max() < 1e-5
+    # integer outputs differ only for splats straddling a float->int cliff
+    assert (a["tiles"] != b["tiles"]).mean() < 1e-3
+    assert (a["meta"] != b["meta"]).any(1).mean() < 1e-2
+
+
+def _raster_one(orc, conic, xy, rgb, W=16, H=16, fp64=False):
+    n = len(xy)
+    vals = np.arange(n, dtype=np.uint32)
+    ranges = np.array([0, n], np.uint32)
+    return orc.raster(W, H, 16, ranges, vals, np.array(conic), np.array(xy), np.array(rgb), fp64=fp64)
+
+
+def test_raster_single_splat_closed_form(orc):
+    """One isotropic splat: C(x,y) = rgb * min(0.99, op * exp(-r^2 / (2 s^2)))
+    where alpha >= 1/255, else 0."""
+    s2 = 4.0
+    op = 0.8
+    img, T, last = _raster_one(orc, [[1 / s2, 0, 1 / s2, op]], [[7.3, 8.6]], [[0.2, 0.5, 1.0, 0]])
+    yy, xx = np.mgrid[0:16, 0:16]
+    a = op * np.exp(-((xx - 7.3) ** 2 + (yy - 8.6) ** 2) / (2 * s2))
+    a = np.minimum(a, 0.99)
+    a = np.where(a >= 1 / 255, a, 0.0)
+    assert np.allclose(img[2], a * 1.0, atol=2e-6)
+    assert np.allclose(img[0], a * 0.2, atol=2e-6)
+    assert np.allclose(T, 1 - a, atol=2e-6)
+    assert np.array_equal(last, (a > 0).astype(np.uint32))
+
+
+def test_raster_opaque_front_stops(orc):
+    """Opaque splats at one pixel: the first blends (alpha clamped to 0.99,
+    T = 0.01); the second would take T to 0.01 * 0.01 < 1e-4, so the pixel stops
+    there without blending it, and the third is never reached."""
+    big = 1e-4
+    img, T, last = _raster_one(orc, [[big, 0, big, 1.0], [big, 0, big, 1.0], [big, 0, big, 1.0]],
+                               [[8, 8], [8, 8], [8, 8]], [[1, 0, 0, 0], [0, 1, 0, 0], [0, 0, 1, 0]])
+    p = (8, 8)
+    assert abs(img[0][p] - 0.99) < 1e-6 and img[1][p] == 0.0 and img[2][p] == 0.0
+    assert last[p] == 1 and abs(T[p] - 0.01) < 1e-6
+
+
+def test_render_fp32_vs_fp64_small(orc):
+    sc, cam = scenes.workload_small()
+    u = cam.uniform(sc.count)
+    a, Ta, la, Ia, _ = orc.render(sc.packed(), sc.count, u)
+    b, Tb, lb, Ib, _ = orc.render(sc.packed(), sc.count, u, fp64=True)
+    assert Ia == Ib == 84916
+    d = np.abs(a - b)
+    assert d.mean() < 1e-5 and (d > 1e-4).mean() < 1e-3

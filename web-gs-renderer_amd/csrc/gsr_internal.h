@@ -1,0 +1,170 @@
+// gsr_internal.h — shared definitions of the gfx950 renderer library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/gsr.h"
+
+namespace gsr {
+
+// ---- device-side constants of the reference pipeline --------------------
+constexpr float kNearView = 0.2f;        // preprocess.wgsl:34
+constexpr float kFrustumGuard = 1.3f;    // preprocess.wgsl:35
+constexpr float kSmall = 1e-7f;          // preprocess.wgsl:32
+constexpr float kAlphaClamp = 0.99f;     // raseter.wgsl:25
+constexpr float kAlphaMin = 1.0f / 255.0f;  // raseter.wgsl:26
+constexpr float kTransmitMin = 0.0001f;  // raseter.wgsl:27
+constexpr uint32_t kParams = 59;         // floats per splat (loader.js:4-11)
+constexpr uint32_t kInstanceFactor = 64; // renderer.js:16
+
+// Planar channel offsets of the [59][N] device layout.
+constexpr int CH_POS = 0, CH_SCALE = 3, CH_ROT = 6, CH_DC = 10, CH_OPACITY = 13, CH_REST = 14;
+
+// exp for FP32, the same operation sequence as the oracle's exp_f32:
+// Cody-Waite reduction x = n ln2 + r, degree-7 Taylor polynomial of e^r in
+// Horner form with fused multiply-adds, scaled by 2^n (v_ldexp_f32).
+// exp_core is valid for x in [-87, 88.7]; gsr_exp adds the range handling.
+__device__ __forceinline__ float exp_core(float x) {
+    const float n = __builtin_rintf(x * 0x1.715476p+0f);
+    float r = __builtin_fmaf(n, -0x1.62e430p-1f, x);
+    r = __builtin_fmaf(n, 0x1.05c610p-29f, r);
+    float p = 0x1.a01a02p-13f;
+    p = __builtin_fmaf(p, r, 0x1.6c16c2p-10f);
+    p = __builtin_fmaf(p, r, 0x1.111112p-7f);
+    p = __builtin_fmaf(p, r, 0x1.555556p-5f);
+    p = __builtin_fmaf(p, r, 0x1.555556p-3f);
+    p = __builtin_fmaf(p, r, 0.5f);
+    p = __builtin_fmaf(p, r, 1.0f);
+    p = __builtin_fmaf(p, r, 1.0f);
+    return __builtin_ldexpf(p, (int)n);
+}
+
+__device__ __forceinline__ float gsr_exp(float x) {
+    if (x != x) return x;
+    if (x > 88.8f) return __builtin_inff();
+    if (x < -87.0f) return 0.0f;
+    return exp_core(x);
+}
+
+// Camera block as the kernels read it (same bytes as gsr_camera).
+struct CamK {
+    float view[16];
+    float view_proj[16];
+    float eye[3];
+    uint32_t count;
+    float focal[2];
+    float tan_fov[2];
+    float texture_size[2];
+    uint32_t pad[2];
+};
+static_assert(sizeof(CamK) == 176, "camera block must be 176 bytes");
+static_assert(sizeof(gsr_camera) == 176, "gsr_camera must be 176 bytes");
+
+// ---- sort tuning ---------------------------------------------------------
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kSortThreads * kSortItems;  // keys per onesweep tile
+constexpr int kRadixBits = 8;
+constexpr int kRadix = 1 << kRadixBits;
+constexpr int kMaxPasses = 8;
+constexpr uint32_t kStatusValueMask = 0x3FFFFFFFu;    // 30-bit counts in look-back words
+constexpr uint64_t kMaxSortKeys = kStatusValueMask;   // sort capacity limit
+
+// Device-resident per-frame counters (one 256-byte block).
+struct FrameCounters {
+    uint64_t instances;       // raw scan total I
+    uint32_t n;               // min(I, capacity)
+    uint32_t error;           // nonzero: a kernel hit a bounded-wait limit
+    uint32_t raster_ticket;   // next (view, tile) work item of the batch raster queue
+    uint32_t pad[64 - 5];
+};
+
+// ---- host-side context -----------------------------------------------------
+struct Scene {
+    float* params = nullptr;   // planar [59][N]
+    uint32_t count = 0;
+};
+
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+struct TimingAcc {
+    std::vector<std::string> order;
+    std::map<std::string, double> sums;
+    int frames = 0;
+};
+
+struct Context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string error;
+    std::map<std::string, Scene> scenes;
+    std::string current;
+    std::map<std::string, DevBuf> bufs;   // grow-only named scratch
+    int cu_count = 256;
+    // timing
+    std::vector<std::pair<std::string, hipEvent_t>> marks;  // label, start event (end = next start)
+    std::vector<hipEvent_t> event_pool;
+    size_t event_used = 0;
+    TimingAcc timing;
+    std::vector<gsr_timing_row> last_rows;
+    int last_frames = 0;
+    bool timing_on = false;
+    // saved forward state for backward
+    bool have_state = false;
+    uint32_t state_count = 0, state_w = 0, state_h = 0, state_tile = 16, state_key_bytes = 4;
+    uint64_t state_capacity = 0;
+    int state_sorted_index = 0;
+    // host pinned staging for small readbacks
+    FrameCounters* host_counters = nullptr;
+};
+
+// Named device buffer, grown to at least `bytes` (contents not preserved).
+void* buffer(Context* c, const char* name, size_t bytes);
+
+int fail(Context* c, int code, const char* fmt, ...);
+#define GSR_HIP(ctx, call)                                                                 \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return ::gsr::fail((ctx), GSR_ERR_DEVICE, "%s failed: %s", #call,              \
+                               hipGetErrorString(e_));                                     \
+    } while (0)
+
+// Timing marks on the context stream (no-ops unless timing is on).
+void mark(Context* c, const char* label);
+void mark_end(Context* c);
+// After the stream has drained: turn the recorded frames' marks into timing
+// rows (per-frame averages over every frame since the last collection).
+void finish_frame_timings(Context* c);
+
+// ---- kernel launchers (each enqueues on c->stream) -------------------------
+int launch_preprocess(Context* c, const float* params, uint32_t n, const CamK& cam, uint32_t tile,
+                      uint32_t* tiles, uint4* meta, float4* conic, float2* xy, float4* rgb);
+int launch_scan(Context* c, const uint32_t* in, uint32_t n, uint32_t* out, FrameCounters* counters,
+                uint64_t capacity);
+int launch_emit(Context* c, const uint32_t* offsets, const uint4* meta, uint32_t n, uint32_t tiles_x,
+                uint64_t capacity, uint32_t key_bytes, void* keys, uint32_t* vals);
+// Sort of n (device counter n_dev if non-null, else n_host) pairs; returns result index.
+int launch_sort(Context* c, uint32_t key_bytes, void* k0, uint32_t* v0, void* k1, uint32_t* v1,
+                const uint32_t* n_dev, uint64_t n_host, uint64_t capacity, uint32_t begin_bit,
+                uint32_t end_bit, FrameCounters* counters, int* result_index);
+int launch_ranges(Context* c, const void* keys, uint32_t key_bytes, const uint32_t* n_dev,
+                  uint64_t n_host, uint32_t num_tiles, uint32_t* ranges, bool clear);
+// splats bounds the per-splat arrays, the instance count (n_dev, else n_host) the ranges.
+int launch_raster(Context* c, uint32_t W, uint32_t H, uint32_t tile, uint32_t splats, const uint32_t* n_dev,
+                  uint32_t n_host, const uint32_t* ranges, const uint32_t* vals, const float4* conic,
+                  const float2* xy, const float4* rgb, float* out, float* tfinal, uint32_t* last);
+int launch_fill_u32(Context* c, uint32_t* p, uint32_t value, size_t count);
+
+__host__ __device__ inline uint32_t ceil_div(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+}  // namespace gsr

@@ -1,0 +1,174 @@
+// preprocess.hip — per-splat projection, 2-D covariance, conic, radius, SH
+// colour, tile rectangle and depth key (src/shaders/preprocess.wgsl:58-284).
+//
+// One thread per splat, 256-thread blocks (4 wave64).  The scene is planar
+// ([59][N] float32), so each of the 59 per-splat loads is one fully coalesced
+// 256-byte wave access.  Spherical harmonics (180 of the 236 bytes) are only
+// read for splats that survive both culls.  FP32 operation order is the
+// oracle's (no contraction, fmaf where written, IEEE sqrt/div, gsr_exp).
+#include "gsr_internal.h"
+
+namespace gsr {
+namespace {
+
+constexpr float SH_C0 = 0.28209479177387814f;
+constexpr float SH_C1 = 0.4886025119029199f;
+constexpr float SH_C2_0 = 1.0925484305920792f, SH_C2_1 = -1.0925484305920792f,
+                SH_C2_2 = 0.31539156525252005f, SH_C2_3 = -1.0925484305920792f,
+                SH_C2_4 = 0.5462742152960396f;
+constexpr float SH_C3_0 = -0.5900435899266435f, SH_C3_1 = 2.890611442640554f,
+                SH_C3_2 = -0.4570457994644658f, SH_C3_3 = 0.3731763325901154f,
+                SH_C3_4 = -0.4570457994644658f, SH_C3_5 = 1.445305721320277f,
+                SH_C3_6 = -0.5900435899266435f;
+
+__global__ __launch_bounds__(256) void k_preprocess(const float* __restrict__ P, uint32_t n, CamK cam,
+                                                    uint32_t tile, uint32_t* __restrict__ tiles_out,
+                                                    uint4* __restrict__ meta_out,
+                                                    float4* __restrict__ conic_out,
+                                                    float2* __restrict__ xy_out,
+                                                    float4* __restrict__ rgb_out) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const size_t N = n;
+    const float px = P[CH_POS * N + g], py = P[(CH_POS + 1) * N + g], pz = P[(CH_POS + 2) * N + g];
+    const float* V = cam.view;
+    const float* PV = cam.view_proj;
+    const float tx = ((V[0] * px + V[4] * py) + V[8] * pz) + V[12];
+    const float ty = ((V[1] * px + V[5] * py) + V[9] * pz) + V[13];
+    const float tz = ((V[2] * px + V[6] * py) + V[10] * pz) + V[14];
+    uint32_t ntiles = 0;
+    uint4 meta = make_uint4(0, 0, 0, 0);
+    float4 conic = make_float4(0.f, 0.f, 0.f, 0.f);
+    float2 xy = make_float2(0.f, 0.f);
+    float4 rgb = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (tz < -kNearView) {
+        const float cx = ((PV[0] * px + PV[4] * py) + PV[8] * pz) + PV[12];
+        const float cy = ((PV[1] * px + PV[5] * py) + PV[9] * pz) + PV[13];
+        const float cz = ((PV[2] * px + PV[6] * py) + PV[10] * pz) + PV[14];
+        const float cw = ((PV[3] * px + PV[7] * py) + PV[11] * pz) + PV[15];
+        const float w = cw + kSmall;
+        const float ndcx = cx / w, ndcy = cy / w, ndcz = cz / w;
+        const float W = cam.texture_size[0], H = cam.texture_size[1];
+        const float pixx = ((ndcx + 1.0f) * W - 1.0f) * 0.5f;
+        const float pixy = ((ndcy + 1.0f) * H - 1.0f) * 0.5f;
+        xy = make_float2(pixx, pixy);
+        const float s0 = gsr_exp(P[(CH_SCALE + 0) * N + g]);
+        const float s1 = gsr_exp(P[(CH_SCALE + 1) * N + g]);
+        const float s2 = gsr_exp(P[(CH_SCALE + 2) * N + g]);
+        float qw = P[(CH_ROT + 0) * N + g], qx = P[(CH_ROT + 1) * N + g];
+        float qy = P[(CH_ROT + 2) * N + g], qz = P[(CH_ROT + 3) * N + g];
+        const float ql = sqrtf(((qx * qx + qy * qy) + qz * qz) + qw * qw);
+        qx = qx / ql; qy = qy / ql; qz = qz / ql; qw = qw / ql;
+        const float x2 = qx + qx, y2 = qy + qy, z2 = qz + qz;
+        const float xx = qx * x2, xy_ = qx * y2, xz = qx * z2;
+        const float yy = qy * y2, yz = qy * z2, zz = qz * z2;
+        const float wx = qw * x2, wy = qw * y2, wz = qw * z2;
+        // R (row, col) as quantToMat3 (preprocess.wgsl:182-201), times diag(s)
+        const float m00 = (1.0f - (yy + zz)) * s0, m01 = (xy_ - wz) * s1, m02 = (xz + wy) * s2;
+        const float m10 = (xy_ + wz) * s0, m11 = (1.0f - (xx + zz)) * s1, m12 = (yz - wx) * s2;
+        const float m20 = (xz - wy) * s0, m21 = (yz + wx) * s1, m22 = (1.0f - (xx + yy)) * s2;
+        // Sigma = M M^T (symmetric)
+        const float S00 = (m00 * m00 + m01 * m01) + m02 * m02;
+        const float S01 = (m00 * m10 + m01 * m11) + m02 * m12;
+        const float S02 = (m00 * m20 + m01 * m21) + m02 * m22;
+        const float S10 = (m10 * m00 + m11 * m01) + m12 * m02;
+        const float S11 = (m10 * m10 + m11 * m11) + m12 * m12;
+        const float S12 = (m10 * m20 + m11 * m21) + m12 * m22;
+        const float S20 = (m20 * m00 + m21 * m01) + m22 * m02;
+        const float S21 = (m20 * m10 + m21 * m11) + m22 * m12;
+        const float S22 = (m20 * m20 + m21 * m21) + m22 * m22;
+        // calculateCovariance2D (preprocess.wgsl:251-269)
+        const float limx = kFrustumGuard * cam.tan_fov[0];
+        const float limy = kFrustumGuard * cam.tan_fov[1];
+        const float ux = fminf(limx, fmaxf(-limx, tx / tz)) * tz;
+        const float uy = fminf(limy, fmaxf(-limy, ty / tz)) * tz;
+        const float uz = tz;
+        const float fx = cam.focal[0], fy = cam.focal[1];
+        const float uz2 = uz * uz;
+        const float J00 = fx / uz, J02 = (-fx * ux) / uz2;
+        const float J11 = fy / uz, J12 = (-fy * uy) / uz2;
+        const float T00 = J00 * V[0] + J02 * V[2], T01 = J00 * V[4] + J02 * V[6], T02 = J00 * V[8] + J02 * V[10];
+        const float T10 = J11 * V[1] + J12 * V[2], T11 = J11 * V[5] + J12 * V[6], T12 = J11 * V[9] + J12 * V[10];
+        const float U00 = (T00 * S00 + T01 * S10) + T02 * S20;
+        const float U01 = (T00 * S01 + T01 * S11) + T02 * S21;
+        const float U02 = (T00 * S02 + T01 * S12) + T02 * S22;
+        const float U10 = (T10 * S00 + T11 * S10) + T12 * S20;
+        const float U11 = (T10 * S01 + T11 * S11) + T12 * S21;
+        const float U12 = (T10 * S02 + T11 * S12) + T12 * S22;
+        const float a = ((U00 * T00 + U01 * T01) + U02 * T02) + 0.3f;
+        const float b = (U00 * T10 + U01 * T11) + U02 * T12;
+        const float c = ((U10 * T10 + U11 * T11) + U12 * T12) + 0.3f;
+        const float det = a * c - b * b;
+        const float mid = 0.5f * (a + c);
+        const float lambda = mid + sqrtf(fmaxf(0.1f, mid * mid - det));
+        const float radius = ceilf(3.0f * sqrtf(lambda));
+        conic = make_float4(c / det, -b / det, a / det,
+                            1.0f / (1.0f + gsr_exp(-P[CH_OPACITY * N + g])));
+        if (!(pixx + radius < 0.0f || pixx - radius > W || pixy + radius < 0.0f || pixy - radius > H)) {
+            // calculateColor (preprocess.wgsl:219-249)
+            float dx = px - cam.eye[0], dy = py - cam.eye[1], dz = pz - cam.eye[2];
+            const float dl = sqrtf((dx * dx + dy * dy) + dz * dz);
+            dx = dx / dl; dy = dy / dl; dz = dz / dl;
+            const float X = dx, Y = dy, Z = dz;
+            const float XX = X * X, YY = Y * Y, ZZ = Z * Z, XY = X * Y, YZ = Y * Z, XZ = X * Z;
+            float basis[15];
+            basis[0] = -(SH_C1 * Y);
+            basis[1] = SH_C1 * Z;
+            basis[2] = -(SH_C1 * X);
+            basis[3] = SH_C2_0 * XY;
+            basis[4] = SH_C2_1 * YZ;
+            basis[5] = SH_C2_2 * ((2.0f * ZZ - XX) - YY);
+            basis[6] = SH_C2_3 * XZ;
+            basis[7] = SH_C2_4 * (XX - YY);
+            basis[8] = (SH_C3_0 * Y) * (3.0f * XX - YY);
+            basis[9] = (SH_C3_1 * XY) * Z;
+            basis[10] = (SH_C3_2 * Y) * ((4.0f * ZZ - XX) - YY);
+            basis[11] = (SH_C3_3 * Z) * ((2.0f * ZZ - 3.0f * XX) - 3.0f * YY);
+            basis[12] = (SH_C3_4 * X) * ((4.0f * ZZ - XX) - YY);
+            basis[13] = (SH_C3_5 * Z) * (XX - YY);
+            basis[14] = (SH_C3_6 * X) * (XX - 3.0f * YY);
+            float col[3];
+#pragma unroll
+            for (int ch = 0; ch < 3; ++ch) {
+                float acc = SH_C0 * P[(CH_DC + ch) * N + g];
+#pragma unroll
+                for (int k = 0; k < 15; ++k)
+                    acc = __builtin_fmaf(basis[k], P[(CH_REST + 15 * ch + k) * N + g], acc);
+                acc = acc + 0.5f;
+                col[ch] = fmaxf(acc, 0.0f);
+            }
+            rgb = make_float4(col[0], col[1], col[2], 0.0f);
+            // tile rectangle, depth16, tiles touched (preprocess.wgsl:122-143)
+            const uint32_t Wi = (uint32_t)W, Hi = (uint32_t)H;
+            const uint32_t TXn = (Wi + tile - 1) / tile, TYn = (Hi + tile - 1) / tile;
+            const float ts = (float)tile;
+            const float hx = (float)(TXn - 1), hy = (float)(TYn - 1);
+            const uint32_t x0 = (uint32_t)fminf(fmaxf(floorf((pixx - radius) / ts), 0.0f), hx);
+            const uint32_t y0 = (uint32_t)fminf(fmaxf(floorf((pixy - radius) / ts), 0.0f), hy);
+            const uint32_t x1 = (uint32_t)fminf(fmaxf(floorf((pixx + radius) / ts), 0.0f), hx);
+            const uint32_t y1 = (uint32_t)fminf(fmaxf(floorf((pixy + radius) / ts), 0.0f), hy);
+            const float zsat = fminf(fmaxf(ndcz, 0.0f), 1.0f);
+            const uint32_t depth16 = (uint32_t)(zsat * 65535.0f);
+            ntiles = (x1 - x0 + 1) * (y1 - y0 + 1);
+            meta = make_uint4((y0 << 16) | x0, (y1 << 16) | x1, depth16, 0u);
+        }
+    }
+    tiles_out[g] = ntiles;
+    meta_out[g] = meta;
+    conic_out[g] = conic;
+    xy_out[g] = xy;
+    rgb_out[g] = rgb;
+}
+
+}  // namespace
+
+int launch_preprocess(Context* c, const float* params, uint32_t n, const CamK& cam, uint32_t tile,
+                      uint32_t* tiles, uint4* meta, float4* conic, float2* xy, float4* rgb) {
+    if (n == 0) return GSR_OK;
+    hipLaunchKernelGGL(k_preprocess, dim3(ceil_div(n, 256)), dim3(256), 0, c->stream, params, n, cam, tile,
+                       tiles, meta, conic, xy, rgb);
+    GSR_HIP(c, hipGetLastError());
+    return GSR_OK;
+}
+
+}  // namespace gsr

@@ -1,0 +1,202 @@
+// raster.hip — tile ranges (src/shaders/tile_ranges.wgsl:7-35) and the
+// front-to-back alpha-blend rasterizer (src/shaders/raseter.wgsl:35-117).
+//
+// Raster layout: one wave64 per tile.  Lane l owns column l % TS and rows
+// l / TS + k * (64 / TS) (1, 4 or 16 pixels for 8, 16, 32-pixel tiles), so a
+// splat's x offset is shared by a lane's pixels.  Each batch of 64 instances
+// is gathered one per lane (value -> conic/opacity, pixel, colour) and
+// broadcast to the wave with v_readlane (no LDS, no barriers); the batch loop
+// stops once every pixel of the tile is saturated, as the reference's
+// per-batch vote does.  Per pixel the FP32 steps are the oracle's:
+//   power = fma(-0.5, fma(A dx, dx, (C dy) dy), -(B dx) dy); skip if > 0;
+//   alpha = min(0.99, opacity * exp(power)); skip if < 1/255;
+//   T' = T (1 - alpha); stop before blending when T' < 1e-4;
+//   rgb += colour * (alpha T); T = T'.
+// A power below -5.55 is skipped before the exp: exp(-5.55) < 1/255 and the
+// opacity is at most 1, so the reference's alpha test would reject it too.
+#include "gsr_internal.h"
+#include "wave.h"
+
+namespace gsr {
+namespace {
+
+template <typename K>
+__global__ __launch_bounds__(256) void k_ranges(const K* __restrict__ keys, const uint32_t* n_dev,
+                                                uint64_t n_host, uint32_t num_tiles,
+                                                uint32_t* __restrict__ ranges) {
+    const uint32_t n = n_dev ? *n_dev : (uint32_t)n_host;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint32_t t = (uint32_t)(keys[i] >> 16);
+        if (t >= num_tiles) continue;
+        if (i == 0) {
+            ranges[2 * t] = 0;
+        } else {
+            const uint32_t p = (uint32_t)(keys[i - 1] >> 16);
+            if (t != p) {
+                if (p < num_tiles) ranges[2 * p + 1] = i;
+                ranges[2 * t] = i;
+            }
+        }
+        if (i == n - 1) ranges[2 * t + 1] = n;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, size_t count) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) p[i] = v;
+}
+
+__device__ __forceinline__ float bcast(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+template <int TS>
+__global__ __launch_bounds__(64) void k_raster(uint32_t W, uint32_t H, uint32_t tiles_x, uint32_t splats,
+                                               const uint32_t* n_dev, uint32_t n_host,
+                                               const uint2* __restrict__ ranges,
+                                               const uint32_t* __restrict__ vals,
+                                               const float4* __restrict__ conic,
+                                               const float2* __restrict__ xy,
+                                               const float4* __restrict__ rgb, float* __restrict__ out,
+                                               float* __restrict__ tfinal, uint32_t* __restrict__ last) {
+    constexpr int PPL = TS * TS / 64;
+    constexpr int STEP = 64 / TS;
+    const uint32_t tile = blockIdx.x;
+    const uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t x = tx * TS + lane % TS;
+    const uint32_t y0 = ty * TS + lane / TS;
+    const float fx = (float)x;
+    float fy[PPL], T[PPL], c0[PPL], c1[PPL], c2[PPL];
+    uint32_t lst[PPL];
+    bool done[PPL];
+#pragma unroll
+    for (int k = 0; k < PPL; ++k) {
+        const uint32_t y = y0 + k * STEP;
+        done[k] = !(x < W && y < H);
+        fy[k] = (float)y;
+        T[k] = 1.0f;
+        c0[k] = c1[k] = c2[k] = 0.0f;
+        lst[k] = 0;
+    }
+    const uint2 r = ranges[tile];
+    const uint32_t n_inst = n_dev ? *n_dev : n_host;
+    const uint32_t start = r.x, end = min(r.y, n_inst);  // ranges never exceed n; bounds the reads
+    for (uint32_t b = start; b < end; b += 64) {
+        bool active = false;
+#pragma unroll
+        for (int k = 0; k < PPL; ++k) active |= !done[k];
+        if (ballot(active) == 0) break;
+        const uint32_t idx = b + lane;
+        float4 co = make_float4(0.f, 0.f, 0.f, 0.f), cl = make_float4(0.f, 0.f, 0.f, 0.f);
+        float2 p = make_float2(0.f, 0.f);
+        if (idx < end) {
+            const uint32_t g = vals[idx];
+            if (g < splats) {
+                co = conic[g];
+                p = xy[g];
+                cl = rgb[g];
+            }
+        }
+        const int m = (int)min(64u, end - b);
+        for (int j = 0; j < m; ++j) {
+            const float gx = bcast(p.x, j), gy = bcast(p.y, j);
+            const float A = bcast(co.x, j), B = bcast(co.y, j), C = bcast(co.z, j), op = bcast(co.w, j);
+            const float cr = bcast(cl.x, j), cg = bcast(cl.y, j), cb = bcast(cl.z, j);
+            const float dx = gx - fx;
+            const float ad = A * dx, bd = B * dx;
+            const uint32_t pos = b - start + (uint32_t)j + 1;
+#pragma unroll
+            for (int k = 0; k < PPL; ++k) {
+                if (done[k]) continue;
+                const float dy = gy - fy[k];
+                const float q = __builtin_fmaf(ad, dx, (C * dy) * dy);
+                const float power = __builtin_fmaf(-0.5f, q, -(bd * dy));
+                if (power > 0.0f || power < -5.55f) continue;
+                const float alpha = fminf(kAlphaClamp, op * exp_core(power));
+                if (alpha < kAlphaMin) continue;
+                const float tn = T[k] * (1.0f - alpha);
+                if (tn < kTransmitMin) {
+                    done[k] = true;
+                    continue;
+                }
+                const float w = alpha * T[k];
+                c0[k] = __builtin_fmaf(cr, w, c0[k]);
+                c1[k] = __builtin_fmaf(cg, w, c1[k]);
+                c2[k] = __builtin_fmaf(cb, w, c2[k]);
+                T[k] = tn;
+                lst[k] = pos;
+            }
+        }
+    }
+    const size_t plane = (size_t)W * H;
+#pragma unroll
+    for (int k = 0; k < PPL; ++k) {
+        const uint32_t y = y0 + k * STEP;
+        if (x < W && y < H) {
+            const size_t o = (size_t)y * W + x;
+            out[o] = c0[k];
+            out[plane + o] = c1[k];
+            out[2 * plane + o] = c2[k];
+            if (tfinal) tfinal[o] = T[k];
+            if (last) last[o] = lst[k];
+        }
+    }
+}
+
+}  // namespace
+
+int launch_fill_u32(Context* c, uint32_t* p, uint32_t value, size_t count) {
+    if (count == 0) return GSR_OK;
+    const uint32_t blocks = (uint32_t)std::min<size_t>((count + 255) / 256, 2048);
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks), dim3(256), 0, c->stream, p, value, count);
+    GSR_HIP(c, hipGetLastError());
+    return GSR_OK;
+}
+
+int launch_ranges(Context* c, const void* keys, uint32_t key_bytes, const uint32_t* n_dev, uint64_t n_host,
+                  uint32_t num_tiles, uint32_t* ranges, bool clear) {
+    if (clear) {
+        int rc = launch_fill_u32(c, ranges, 0u, (size_t)num_tiles * 2);
+        if (rc) return rc;
+    }
+    const uint32_t blocks = (uint32_t)c->cu_count * 4;
+    if (key_bytes == 8)
+        hipLaunchKernelGGL(k_ranges<uint64_t>, dim3(blocks), dim3(256), 0, c->stream, (const uint64_t*)keys,
+                           n_dev, n_host, num_tiles, ranges);
+    else
+        hipLaunchKernelGGL(k_ranges<uint32_t>, dim3(blocks), dim3(256), 0, c->stream, (const uint32_t*)keys,
+                           n_dev, n_host, num_tiles, ranges);
+    GSR_HIP(c, hipGetLastError());
+    return GSR_OK;
+}
+
+int launch_raster(Context* c, uint32_t W, uint32_t H, uint32_t tile, uint32_t splats, const uint32_t* n_dev,
+                  uint32_t n_host, const uint32_t* ranges, const uint32_t* vals, const float4* conic,
+                  const float2* xy, const float4* rgb, float* out, float* tfinal, uint32_t* last) {
+    const uint32_t tx = ceil_div(W, tile), ty = ceil_div(H, tile);
+    const uint32_t tiles = tx * ty;
+    if (tiles == 0) return GSR_OK;
+    const uint2* rg = (const uint2*)ranges;
+    switch (tile) {
+        case 8:
+            hipLaunchKernelGGL(k_raster<8>, dim3(tiles), dim3(64), 0, c->stream, W, H, tx, splats, n_dev, n_host, rg, vals, conic, xy,
+                               rgb, out, tfinal, last);
+            break;
+        case 16:
+            hipLaunchKernelGGL(k_raster<16>, dim3(tiles), dim3(64), 0, c->stream, W, H, tx, splats, n_dev, n_host, rg, vals, conic, xy,
+                               rgb, out, tfinal, last);
+            break;
+        case 32:
+            hipLaunchKernelGGL(k_raster<32>, dim3(tiles), dim3(64), 0, c->stream, W, H, tx, splats, n_dev, n_host, rg, vals, conic, xy,
+                               rgb, out, tfinal, last);
+            break;
+        default:
+            return fail(c, GSR_ERR_ARGUMENT, "tile size %u not in {8, 16, 32}", tile);
+    }
+    GSR_HIP(c, hipGetLastError());
+    return GSR_OK;
+}
+
+}  // namespace gsr

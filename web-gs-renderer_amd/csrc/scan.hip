@@ -1,0 +1,109 @@
+// scan.hip — exclusive scan of the per-splat tile counts into instance
+// offsets (src/shaders/offset_scan.wgsl:22-68), plus the device-side sizing
+// of src/shaders/indirect_arg.wgsl:14-28 (n = min(I, capacity)).
+//
+// Single pass with decoupled look-back: 256-thread blocks take 4096-entry
+// tiles in ticket order; each thread sums 16 entries, waves scan with DPP,
+// and one lane per block chains the tile prefix through 64-bit status words
+// (flag in bits 62-63, running total in bits 0-61), so totals beyond 2^32 are
+// detected rather than wrapped.
+#include "gsr_internal.h"
+#include "wave.h"
+
+namespace gsr {
+namespace {
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kScanThreads * kScanItems;
+constexpr uint64_t kFlagAgg = 1ull << 62, kFlagInc = 2ull << 62, kValMask = (1ull << 62) - 1;
+
+// counts[0..n) -> out[0..n], out[n] = total.  status[0] is the tile ticket,
+// status[1 + t] the look-back word of tile t; all zeroed before the launch.
+__global__ __launch_bounds__(kScanThreads) void k_scan(const uint32_t* in, uint32_t n,
+                                                       uint32_t* out,  // may alias in
+                                                       uint64_t* __restrict__ status_base,
+                                                       FrameCounters* counters, uint64_t capacity) {
+    uint64_t* status = status_base + 1;
+    __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_wave[kScanThreads / 64];
+    __shared__ uint64_t s_prefix;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) s_tile = atomicAdd((uint32_t*)status_base, 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint64_t total_len = (uint64_t)n + 1;  // entry n is the virtual 0 giving the total
+    const uint32_t num_tiles = ceil_div(total_len, kScanTile);
+    const uint64_t base = (uint64_t)tile * kScanTile + (uint64_t)tid * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        const uint64_t p = base + i;
+        v[i] = p < n ? in[p] : 0u;
+        sum += v[i];
+    }
+    const uint32_t incl = wave_inclusive_sum(sum);
+    if (lane == 63) s_wave[wave] = incl;
+    __syncthreads();
+    uint32_t wave_prefix = 0, block_total = 0;
+#pragma unroll
+    for (int w = 0; w < kScanThreads / 64; ++w) {
+        const uint32_t t = s_wave[w];
+        if (w < (int)wave) wave_prefix += t;
+        block_total += t;
+    }
+    if (tid == 0) {
+        uint64_t excl = 0;
+        if (tile == 0) {
+            st_release_u64(&status[0], kFlagInc | block_total);
+        } else {
+            st_release_u64(&status[tile], kFlagAgg | block_total);
+            int64_t j = (int64_t)tile - 1;
+            uint32_t spins = 0;
+            while (j >= 0) {
+                const uint64_t s = ld_acquire_u64(&status[j]);
+                if ((s >> 62) == 0) {
+                    if (++spins > kSpinLimit) { atomicOr(&counters->error, 1u); break; }
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                excl += s & kValMask;
+                if ((s & kFlagInc) != 0) break;
+                --j;
+            }
+            st_release_u64(&status[tile], kFlagInc | ((excl + block_total) & kValMask));
+        }
+        s_prefix = excl;
+        if (tile == num_tiles - 1) {
+            const uint64_t total = excl + block_total;
+            counters->instances = total;
+            counters->n = (uint32_t)(total < capacity ? total : capacity);
+        }
+    }
+    __syncthreads();
+    uint64_t run = s_prefix + wave_prefix + (incl - sum);
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        const uint64_t p = base + i;
+        if (p <= n) out[p] = (uint32_t)run;
+        run += v[i];
+    }
+}
+
+}  // namespace
+
+int launch_scan(Context* c, const uint32_t* in, uint32_t n, uint32_t* out, FrameCounters* counters,
+                uint64_t capacity) {
+    const uint32_t tiles = ceil_div((uint64_t)n + 1, kScanTile);
+    uint64_t* status = (uint64_t*)buffer(c, "scan_status", (size_t)(tiles + 1) * 8);
+    if (!status) return GSR_ERR_ALLOC;
+    int rc = launch_fill_u32(c, (uint32_t*)status, 0u, (size_t)(tiles + 1) * 2);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_scan, dim3(tiles), dim3(kScanThreads), 0, c->stream, in, n, out, status, counters,
+                       capacity);
+    GSR_HIP(c, hipGetLastError());
+    return GSR_OK;
+}
+
+}  // namespace gsr
