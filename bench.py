@@ -154,6 +154,29 @@ def main():
         "sort_gbs": round(algorithmic_bytes("sort", n, instances, visible, W, H, tiles, key_bytes, passes)
                           / (sort_ms * 1e-3) / 1e9, 2) if sort_ms else None,
     }
+    # forward + backward (config: same scene, upstream dL/dRGB ~ N(0,1) resident in HBM)
+    g = torch.Generator(device="cuda").manual_seed(2)
+    dldc = torch.randn(3 * H * W, device="cuda", generator=g)
+    grad = torch.empty((59, n), dtype=torch.float32, device="cuda")
+    sopts = lib.options(tile_size=16, save_state=True)
+    for _ in range(max(2, args.warmup // 2)):
+        ctx.forward(params, n, u, sopts, None)
+        ctx.backward(params, n, u, dldc, grad)
+    ctx.synchronize()
+    bsteps = max(10, args.steps // 2)
+    tsopts = lib.options(tile_size=16, save_state=True, timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(bsteps):
+        ctx.forward(params, n, u, tsopts, None)
+        ctx.backward(params, n, u, dldc, grad)
+    ctx.synchronize()
+    dtb = time.perf_counter() - t0
+    bw_ms = ctx.timings()
+    extra["fwd_bwd_frames_per_s"] = round(bsteps / dtb, 3)
+    extra["fwd_bwd_ms_per_step"] = round(dtb * 1000.0 / bsteps, 5)
+    extra["fwd_bwd_stage_ms"] = {k: round(v, 5) for k, v in bw_ms.items()}
+    del dldc, grad
     cpu = None
     if not args.no_cpu:
         log("cpu baseline")

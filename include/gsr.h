@@ -124,7 +124,9 @@ int gsr_last_instance_count(gsr_context* ctx, uint64_t* instances);
 int gsr_get_timings(gsr_context* ctx, gsr_timing_row* rows, int capacity, int* count);
 
 /* rgba8 conversion with WebGPU unorm8 semantics (raseter.wgsl:19,109): clamp to
- * [0,1], x255, round to nearest; alpha 255.  flip_y = 1 applies blit.wgsl:22. */
+ * [0,1], x255, round to nearest; alpha 255.  flip_y = 1 applies blit.wgsl:22.
+ * rgb_dev == NULL converts the context's last rendered frame (which must be
+ * width x height). */
 int gsr_to_rgba8(gsr_context* ctx, const float* rgb_dev, uint32_t width, uint32_t height,
                  int flip_y, uint8_t* out_rgba_host);
 
@@ -178,10 +180,12 @@ int gsr_rasterize(gsr_context* ctx, uint32_t width, uint32_t height, uint32_t ti
 /* ---- differentiation (no reference counterpart: the reference is forward-only) ---- */
 /*
  * Gradients of the last gsr_forward (which must have had save_state = 1,
- * and the same params/camera) with respect to the planar parameters.
+ * and the same params/count/camera) with respect to the planar parameters.
  * dL_drgb_dev: f32[3][H][W].  grad_params_dev: f32[59][N], overwritten.
- * Optional per-splat 2-D gradients (f32[N][8]: xy(2), conic(3), opacity(1),
- * pad) can be returned for inspection through grad2d_dev (may be NULL).
+ * grad2d_dev (may be NULL) receives the per-splat 2-D gradient, f32[N][12]:
+ * pixel x, y; conic A, B, C; opacity; r, g, b; 3 unused.  Enqueues on the
+ * context stream like gsr_forward.  Gradients accumulate with float atomics,
+ * so the last bits may differ between runs.
  */
 int gsr_backward(gsr_context* ctx, const float* params_dev, uint32_t count,
                  const gsr_camera* cam, const float* dL_drgb_dev, float* grad_params_dev,

@@ -382,9 +382,21 @@ void tile_ranges(const uint64_t* keys, uint64_t n, uint32_t num_tiles, uint32_t*
 // raseter.wgsl:35-117 for every pixel: integer pixel coordinates, blend in
 // range order, alpha clamp 0.99, cutoff 1/255, stop before T < 1e-4.
 // last[] is the 1-based range position of the last blended splat.
+inline uint64_t fnv(uint64_t h, uint64_t v) {
+    for (int k = 0; k < 8; ++k) {
+        h ^= (v >> (8 * k)) & 0xFF;
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+// sig (optional, per pixel): hash of the pixel's discrete decisions (which
+// splats it blends and where it stops), to tell when two frames lie on the
+// same smooth branch of the renderer.
 template <typename T>
 void raster_all(uint32_t W, uint32_t H, uint32_t tile, const uint32_t* ranges, const uint32_t* vals,
-                const T* conic, const T* xy, const T* rgb, T* out, T* tfinal, uint32_t* last) {
+                const T* conic, const T* xy, const T* rgb, T* out, T* tfinal, uint32_t* last,
+                uint64_t* sig = nullptr) {
     const uint32_t TXn = (W + tile - 1) / tile, TYn = (H + tile - 1) / tile;
     const int64_t ntile = (int64_t)TXn * TYn;
 #pragma omp parallel for schedule(dynamic, 1)
@@ -398,6 +410,7 @@ void raster_all(uint32_t W, uint32_t H, uint32_t tile, const uint32_t* ranges, c
                 const T fxp = T(x), fyp = T(y);
                 T Tr = 1, c0 = 0, c1 = 0, c2 = 0;
                 uint32_t lastc = 0;
+                uint64_t h = 1469598103934665603ull;
                 for (uint32_t i = start; i < end; ++i) {
                     const uint32_t g = vals[i];
                     const T A = conic[g * 4], B = conic[g * 4 + 1], C = conic[g * 4 + 2];
@@ -410,7 +423,11 @@ void raster_all(uint32_t W, uint32_t H, uint32_t tile, const uint32_t* ranges, c
                     const T alpha = std::min(T(0.99f), op * M<T>::ex(power));
                     if (alpha < T(1.0f / 255.0f)) continue;
                     const T testT = Tr * (T(1) - alpha);
-                    if (testT < T(0.0001f)) break;
+                    if (testT < T(0.0001f)) {
+                        h = fnv(h, 0xFFFFFFFFull + i);
+                        break;
+                    }
+                    h = fnv(h, ((uint64_t)vals[i] << 1) | (op * M<T>::ex(power) > T(0.99f) ? 1u : 0u));
                     const T wgt = alpha * Tr;
                     c0 = M<T>::fma(rgb[g * 4], wgt, c0);
                     c1 = M<T>::fma(rgb[g * 4 + 1], wgt, c1);
@@ -424,13 +441,15 @@ void raster_all(uint32_t W, uint32_t H, uint32_t tile, const uint32_t* ranges, c
                 out[(size_t)2 * W * H + p] = c2;
                 if (tfinal) tfinal[p] = Tr;
                 if (last) last[p] = lastc;
+                if (sig) sig[p] = h;
             }
     }
 }
 
 template <typename T>
 int render(const float* packed, uint32_t n, const Cam& cam, uint32_t tile, uint64_t cap, T* out,
-           T* tfinal, uint32_t* last, uint64_t* instances, double* stage_ms, int use_radix) {
+           T* tfinal, uint32_t* last, uint64_t* instances, double* stage_ms, int use_radix,
+           uint64_t* signature = nullptr) {
     using clk = std::chrono::steady_clock;
     auto ms = [](clk::time_point a, clk::time_point b) {
         return std::chrono::duration<double, std::milli>(b - a).count();
@@ -462,9 +481,16 @@ int render(const float* packed, uint32_t n, const Cam& cam, uint32_t tile, uint6
     std::vector<uint32_t> ranges((size_t)TXn * TYn * 2);
     tile_ranges(keys.data(), m, TXn * TYn, ranges.data());
     auto t5 = clk::now();
+    std::vector<uint64_t> psig(signature ? (size_t)W * H : 0);
     raster_all<T>(W, H, tile, ranges.data(), vals.data(), conic.data(), xy.data(), rgb.data(), out,
-                  tfinal, last);
+                  tfinal, last, signature ? psig.data() : nullptr);
     auto t6 = clk::now();
+    if (signature) {
+        uint64_t h = fnv(1469598103934665603ull, total);
+        for (uint64_t i = 0; i < m; ++i) h = fnv(h, keys[i] ^ ((uint64_t)vals[i] << 40));
+        for (uint64_t v : psig) h = fnv(h, v);
+        *signature = h;
+    }
     if (instances) *instances = total;
     if (stage_ms) {
         stage_ms[0] = ms(t0, t1);
@@ -477,6 +503,346 @@ int render(const float* packed, uint32_t n, const Cam& cam, uint32_t tile, uint6
     return total > cap ? 1 : 0;
 }
 
+
+// ---------------------------------------------------------------------------
+// FP64 adjoint of the frame (the reference has no backward; this is the
+// analytic derivative of the forward above).  Gradient layout per splat
+// (grad2d, 12 doubles): d/d(x, y) of the pixel position, d/d(A, B, C) of the
+// conic, d/d opacity, d/d(r, g, b), 3 unused.  Parameter gradients are
+// planar [59][N] like the device layout.
+
+constexpr int G2 = 12;
+
+// Per-splat preprocess adjoint (chain of preprocess.wgsl:80-135 in FP64).
+// Writes d/d(parameter ch) to out[ch * stride + index].
+void preprocess_backward_one(const Packed& P, uint32_t g, const Cam& cam, const double* G, double* out,
+                             size_t stride, size_t index) {
+    auto put = [&](int ch, double v) { out[(size_t)ch * stride + index] = v; };
+    const double px = P.pos(g, 0), py = P.pos(g, 1), pz = P.pos(g, 2);
+    const float* V = cam.view;
+    const float* PV = cam.view_proj;
+    const double tx = V[0] * px + V[4] * py + V[8] * pz + V[12];
+    const double ty = V[1] * px + V[5] * py + V[9] * pz + V[13];
+    const double tz = V[2] * px + V[6] * py + V[10] * pz + V[14];
+    double gp[3] = {0, 0, 0};
+    bool any = false;
+    for (int k = 0; k < 9; ++k) any |= G[k] != 0.0;
+    if (!any || tz >= -0.2) {
+        for (int ch = 0; ch < 59; ++ch) put(ch, 0.0);
+        return;
+    }
+    // pixel position <- clip <- p
+    const double cx = PV[0] * px + PV[4] * py + PV[8] * pz + PV[12];
+    const double cy = PV[1] * px + PV[5] * py + PV[9] * pz + PV[13];
+    const double cw = PV[3] * px + PV[7] * py + PV[11] * pz + PV[15];
+    const double w = cw + double(1e-7f);
+    const double W = cam.texture_size[0], H = cam.texture_size[1];
+    const double gnx = G[0] * W * 0.5, gny = G[1] * H * 0.5;
+    const double gcx = gnx / w, gcy = gny / w, gcw = -(gnx * cx + gny * cy) / (w * w);
+    for (int c = 0; c < 3; ++c) gp[c] += PV[c * 4 + 0] * gcx + PV[c * 4 + 1] * gcy + PV[c * 4 + 3] * gcw;
+    // forward intermediates of the covariance
+    const double s[3] = {std::exp((double)P.lscale(g, 0)), std::exp((double)P.lscale(g, 1)),
+                         std::exp((double)P.lscale(g, 2))};
+    const double rq[4] = {P.rot(g, 0), P.rot(g, 1), P.rot(g, 2), P.rot(g, 3)};  // w x y z
+    const double ql = std::sqrt(rq[0] * rq[0] + rq[1] * rq[1] + rq[2] * rq[2] + rq[3] * rq[3]);
+    const double qw = rq[0] / ql, qx = rq[1] / ql, qy = rq[2] / ql, qz = rq[3] / ql;
+    double R[3][3];
+    R[0][0] = 1 - 2 * (qy * qy + qz * qz); R[0][1] = 2 * (qx * qy - qw * qz); R[0][2] = 2 * (qx * qz + qw * qy);
+    R[1][0] = 2 * (qx * qy + qw * qz); R[1][1] = 1 - 2 * (qx * qx + qz * qz); R[1][2] = 2 * (qy * qz - qw * qx);
+    R[2][0] = 2 * (qx * qz - qw * qy); R[2][1] = 2 * (qy * qz + qw * qx); R[2][2] = 1 - 2 * (qx * qx + qy * qy);
+    double Mm[3][3], S[3][3];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) Mm[r][c] = R[r][c] * s[c];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) S[i][j] = Mm[i][0] * Mm[j][0] + Mm[i][1] * Mm[j][1] + Mm[i][2] * Mm[j][2];
+    const double limx = double(1.3f) * cam.tan_fov[0], limy = double(1.3f) * cam.tan_fov[1];
+    const double vx = tx / tz, vy = ty / tz;
+    const bool clx = vx < -limx || vx > limx, cly = vy < -limy || vy > limy;
+    const double ux = std::min(limx, std::max(-limx, vx)) * tz;
+    const double uy = std::min(limy, std::max(-limy, vy)) * tz;
+    const double uz = tz;
+    const double fx = cam.focal[0], fy = cam.focal[1];
+    const double J00 = fx / uz, J02 = -fx * ux / (uz * uz), J11 = fy / uz, J12 = -fy * uy / (uz * uz);
+    double Tm[2][3];
+    for (int c = 0; c < 3; ++c) {
+        Tm[0][c] = J00 * V[c * 4 + 0] + J02 * V[c * 4 + 2];
+        Tm[1][c] = J11 * V[c * 4 + 1] + J12 * V[c * 4 + 2];
+    }
+    double St0[3], St1[3];
+    for (int i = 0; i < 3; ++i) {
+        St0[i] = S[i][0] * Tm[0][0] + S[i][1] * Tm[0][1] + S[i][2] * Tm[0][2];
+        St1[i] = S[i][0] * Tm[1][0] + S[i][1] * Tm[1][1] + S[i][2] * Tm[1][2];
+    }
+    const double a = Tm[0][0] * St0[0] + Tm[0][1] * St0[1] + Tm[0][2] * St0[2] + double(0.3f);
+    const double b = Tm[0][0] * St1[0] + Tm[0][1] * St1[1] + Tm[0][2] * St1[2];
+    const double c = Tm[1][0] * St1[0] + Tm[1][1] * St1[1] + Tm[1][2] * St1[2] + double(0.3f);
+    const double det = a * c - b * b, det2 = det * det;
+    // conic (A, B, C) = (c, -b, a) / det
+    const double gA = G[2], gB = G[3], gC = G[4];
+    const double ga = gA * (-c * c / det2) + gB * (b * c / det2) + gC * (-b * b / det2);
+    const double gb = gA * (2 * b * c / det2) + gB * (-(det + 2 * b * b) / det2) + gC * (2 * a * b / det2);
+    const double gc = gA * (-b * b / det2) + gB * (a * b / det2) + gC * (-a * a / det2);
+    // (a, b, c) = (t0' S t0, t0' S t1, t1' S t1) with t0, t1 the rows of T = J W
+    double gt0[3], gt1[3], gS[3][3];
+    for (int i = 0; i < 3; ++i) {
+        gt0[i] = 2 * ga * St0[i] + gb * St1[i];
+        gt1[i] = gb * St0[i] + 2 * gc * St1[i];
+    }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            gS[i][j] = ga * Tm[0][i] * Tm[0][j] + gb * Tm[0][i] * Tm[1][j] + gc * Tm[1][i] * Tm[1][j];
+    double gJ00 = 0, gJ02 = 0, gJ11 = 0, gJ12 = 0;
+    for (int cc = 0; cc < 3; ++cc) {
+        gJ00 += gt0[cc] * V[cc * 4 + 0];
+        gJ02 += gt0[cc] * V[cc * 4 + 2];
+        gJ11 += gt1[cc] * V[cc * 4 + 1];
+        gJ12 += gt1[cc] * V[cc * 4 + 2];
+    }
+    const double uz2 = uz * uz, uz3 = uz2 * uz;
+    const double gux = gJ02 * (-fx / uz2), guy = gJ12 * (-fy / uz2);
+    const double guz = gJ00 * (-fx / uz2) + gJ02 * (2 * fx * ux / uz3) + gJ11 * (-fy / uz2) + gJ12 * (2 * fy * uy / uz3);
+    double gt[3] = {clx ? 0.0 : gux, cly ? 0.0 : guy, guz};
+    if (clx) gt[2] += gux * (vx > 0 ? limx : -limx);
+    if (cly) gt[2] += guy * (vy > 0 ? limy : -limy);
+    for (int cc = 0; cc < 3; ++cc) gp[cc] += V[cc * 4 + 0] * gt[0] + V[cc * 4 + 1] * gt[1] + V[cc * 4 + 2] * gt[2];
+    // Sigma = M M'  ->  dM = (gS + gS') M
+    double gM[3][3];
+    for (int r = 0; r < 3; ++r)
+        for (int cc = 0; cc < 3; ++cc) {
+            double acc = 0;
+            for (int k = 0; k < 3; ++k) acc += (gS[r][k] + gS[k][r]) * Mm[k][cc];
+            gM[r][cc] = acc;
+        }
+    double gR[3][3], gls[3];
+    for (int cc = 0; cc < 3; ++cc) {
+        double gsc = 0;
+        for (int r = 0; r < 3; ++r) {
+            gR[r][cc] = gM[r][cc] * s[cc];
+            gsc += gM[r][cc] * R[r][cc];
+        }
+        gls[cc] = gsc * s[cc];
+    }
+    // R(q) for the normalised quaternion, then through the normalisation
+    const double gqw = 2 * (-qz * gR[0][1] + qy * gR[0][2] + qz * gR[1][0] - qx * gR[1][2] - qy * gR[2][0] + qx * gR[2][1]);
+    const double gqx = 2 * (qy * gR[0][1] + qz * gR[0][2] + qy * gR[1][0] - 2 * qx * gR[1][1] - qw * gR[1][2] +
+                            qz * gR[2][0] + qw * gR[2][1] - 2 * qx * gR[2][2]);
+    const double gqy = 2 * (-2 * qy * gR[0][0] + qx * gR[0][1] + qw * gR[0][2] + qx * gR[1][0] + qz * gR[1][2] -
+                            qw * gR[2][0] + qz * gR[2][1] - 2 * qy * gR[2][2]);
+    const double gqz = 2 * (-2 * qz * gR[0][0] - qw * gR[0][1] + qx * gR[0][2] + qw * gR[1][0] - 2 * qz * gR[1][1] +
+                            qy * gR[1][2] + qx * gR[2][0] + qy * gR[2][1]);
+    const double qh[4] = {qw, qx, qy, qz}, gqh[4] = {gqw, gqx, gqy, gqz};
+    const double dotq = qh[0] * gqh[0] + qh[1] * gqh[1] + qh[2] * gqh[2] + qh[3] * gqh[3];
+    for (int k = 0; k < 4; ++k) put(6 + k, (gqh[k] - qh[k] * dotq) / ql);
+    for (int k = 0; k < 3; ++k) put(3 + k, gls[k]);
+    // opacity = sigmoid(logit)
+    const double op = 1.0 / (1.0 + std::exp(-(double)P.opacity(g)));
+    put(13, G[5] * op * (1 - op));
+    // colour: SH degree 3 at d = normalize(p - eye), clamped at 0
+    double v3[3] = {px - cam.eye[0], py - cam.eye[1], pz - cam.eye[2]};
+    const double vl = std::sqrt(v3[0] * v3[0] + v3[1] * v3[1] + v3[2] * v3[2]);
+    const double X = v3[0] / vl, Y = v3[1] / vl, Z = v3[2] / vl;
+    const double XX = X * X, YY = Y * Y, ZZ = Z * Z;
+    const double C1 = float(SH_C1);
+    const double C2[5] = {float(SH_C2[0]), float(SH_C2[1]), float(SH_C2[2]), float(SH_C2[3]), float(SH_C2[4])};
+    const double C3[7] = {float(SH_C3[0]), float(SH_C3[1]), float(SH_C3[2]), float(SH_C3[3]),
+                          float(SH_C3[4]), float(SH_C3[5]), float(SH_C3[6])};
+    const double basis[15] = {-C1 * Y, C1 * Z, -C1 * X, C2[0] * X * Y, C2[1] * Y * Z,
+                              C2[2] * (2 * ZZ - XX - YY), C2[3] * X * Z, C2[4] * (XX - YY),
+                              C3[0] * Y * (3 * XX - YY), C3[1] * X * Y * Z, C3[2] * Y * (4 * ZZ - XX - YY),
+                              C3[3] * Z * (2 * ZZ - 3 * XX - 3 * YY), C3[4] * X * (4 * ZZ - XX - YY),
+                              C3[5] * Z * (XX - YY), C3[6] * X * (XX - 3 * YY)};
+    const double dbx[15] = {0, 0, -C1, C2[0] * Y, 0, C2[2] * (-2 * X), C2[3] * Z, C2[4] * 2 * X,
+                            C3[0] * Y * 6 * X, C3[1] * Y * Z, C3[2] * Y * (-2 * X), C3[3] * Z * (-6 * X),
+                            C3[4] * (4 * ZZ - 3 * XX - YY), C3[5] * Z * 2 * X, C3[6] * (3 * XX - 3 * YY)};
+    const double dby[15] = {-C1, 0, 0, C2[0] * X, C2[1] * Z, C2[2] * (-2 * Y), 0, C2[4] * (-2 * Y),
+                            C3[0] * (3 * XX - 3 * YY), C3[1] * X * Z, C3[2] * (4 * ZZ - XX - 3 * YY),
+                            C3[3] * Z * (-6 * Y), C3[4] * X * (-2 * Y), C3[5] * Z * (-2 * Y), C3[6] * X * (-6 * Y)};
+    const double dbz[15] = {0, C1, 0, 0, C2[1] * Y, C2[2] * 4 * Z, C2[3] * X, 0, 0, C3[1] * X * Y,
+                            C3[2] * Y * 8 * Z, C3[3] * (6 * ZZ - 3 * XX - 3 * YY), C3[4] * X * 8 * Z,
+                            C3[5] * (XX - YY), 0};
+    const double C0 = float(SH_C0);
+    double gd[3] = {0, 0, 0};
+    for (int ch = 0; ch < 3; ++ch) {
+        double acc = C0 * P.dc(g, ch);
+        for (int k = 0; k < 15; ++k) acc += basis[k] * P.rest(g, k, ch);
+        const double gch = (acc + 0.5 < 0.0) ? 0.0 : G[6 + ch];
+        put(10 + ch, C0 * gch);
+        for (int k = 0; k < 15; ++k) {
+            const double f = P.rest(g, k, ch);
+            put(14 + 15 * ch + k, basis[k] * gch);
+            gd[0] += gch * dbx[k] * f;
+            gd[1] += gch * dby[k] * f;
+            gd[2] += gch * dbz[k] * f;
+        }
+    }
+    const double dd = X * gd[0] + Y * gd[1] + Z * gd[2];
+    gp[0] += (gd[0] - X * dd) / vl;
+    gp[1] += (gd[1] - Y * dd) / vl;
+    gp[2] += (gd[2] - Z * dd) / vl;
+    for (int k = 0; k < 3; ++k) put(k, gp[k]);
+}
+
+// Whole-frame FP64 forward + adjoint.  decisions_f32 = 1 takes every
+// discrete decision (tile rectangles, sort order, which splats a pixel
+// blends, where it stops) from the FP32 pipeline and evaluates the values in
+// FP64: the adjoint of the FP32 forward's branch.  decisions_f32 = 0 is the
+// pure FP64 frame (what finite differences of orc_render_f64 see).
+void backward_f64(const float* packed, uint32_t n, const Cam& cam, uint32_t tile, const double* dLdC,
+                  int decisions_f32, double* grad, double* grad2d, double* image, double* grad2d_mass) {
+    const uint32_t W = (uint32_t)cam.texture_size[0], H = (uint32_t)cam.texture_size[1];
+    const uint32_t TXn = (W + tile - 1) / tile, TYn = (H + tile - 1) / tile;
+    const size_t N = n;
+    std::vector<uint32_t> tiles(N), meta(N * 4);
+    std::vector<double> conic(N * 4), xy(N * 2), rgb(N * 4);
+    std::vector<float> conicf(N * 4), xyf(N * 2), rgbf(N * 4);
+    {
+        std::vector<uint32_t> t64(N), m64(N * 4);
+        preprocess_all<double>(packed, n, cam, tile, t64.data(), m64.data(), conic.data(), xy.data(), rgb.data());
+        if (decisions_f32) {
+            preprocess_all<float>(packed, n, cam, tile, tiles.data(), meta.data(), conicf.data(), xyf.data(),
+                                  rgbf.data());
+        } else {
+            tiles = t64;
+            meta = m64;
+        }
+    }
+    std::vector<uint32_t> counts(tiles), offsets(N + 1);
+    counts.push_back(0);
+    const uint64_t total = scan_exclusive(counts.data(), n + 1, offsets.data());
+    std::vector<uint64_t> keys(total);
+    std::vector<uint32_t> vals(total);
+    emit_all(offsets.data(), meta.data(), n, W, tile, ~0ull, keys.data(), vals.data());
+    uint32_t tbits = 0;
+    while ((1ull << tbits) < (uint64_t)TXn * TYn) ++tbits;
+    sort_radix_parallel(keys.data(), vals.data(), total, 16 + tbits);
+    std::vector<uint32_t> ranges((size_t)TXn * TYn * 2);
+    tile_ranges(keys.data(), total, TXn * TYn, ranges.data());
+    std::fill(grad2d, grad2d + N * G2, 0.0);
+    if (grad2d_mass) std::fill(grad2d_mass, grad2d_mass + N * G2, 0.0);
+    if (image) std::fill(image, image + (size_t)3 * W * H, 0.0);
+    const int64_t ntile = (int64_t)TXn * TYn;
+#pragma omp parallel
+    {
+        std::vector<double> acc, accm;
+        std::vector<uint32_t> blend;
+        std::vector<double> al, Ts;
+#pragma omp for schedule(dynamic, 1)
+        for (int64_t t = 0; t < ntile; ++t) {
+            const uint32_t tx = (uint32_t)(t % TXn), ty = (uint32_t)(t / TXn);
+            const uint32_t start = ranges[t * 2], end = ranges[t * 2 + 1];
+            if (end <= start) continue;
+            acc.assign((size_t)(end - start) * G2, 0.0);
+            accm.assign((size_t)(end - start) * G2, 0.0);
+            for (uint32_t ly = 0; ly < tile; ++ly)
+                for (uint32_t lx = 0; lx < tile; ++lx) {
+                    const uint32_t x = tx * tile + lx, y = ty * tile + ly;
+                    if (x >= W || y >= H) continue;
+                    // discrete decisions: which range positions this pixel blends
+                    blend.clear();
+                    if (decisions_f32) {
+                        float Tr = 1.0f;
+                        for (uint32_t i = start; i < end; ++i) {
+                            const uint32_t g = vals[i];
+                            const float A = conicf[g * 4], B = conicf[g * 4 + 1], C = conicf[g * 4 + 2];
+                            const float dx = xyf[g * 2] - (float)x, dy = xyf[g * 2 + 1] - (float)y;
+                            const float q = fmaf(A * dx, dx, (C * dy) * dy);
+                            const float power = fmaf(-0.5f, q, -((B * dx) * dy));
+                            if (power > 0.0f) continue;
+                            const float alpha = std::min(0.99f, conicf[g * 4 + 3] * exp_f32(power));
+                            if (alpha < 1.0f / 255.0f) continue;
+                            const float tn = Tr * (1.0f - alpha);
+                            if (tn < 0.0001f) break;
+                            Tr = tn;
+                            blend.push_back(i);
+                        }
+                    } else {
+                        double Tr = 1.0;
+                        for (uint32_t i = start; i < end; ++i) {
+                            const uint32_t g = vals[i];
+                            const double dx = xy[g * 2] - x, dy = xy[g * 2 + 1] - y;
+                            const double power = -0.5 * (conic[g * 4] * dx * dx + conic[g * 4 + 2] * dy * dy) -
+                                                 conic[g * 4 + 1] * dx * dy;
+                            if (power > 0.0) continue;
+                            const double alpha = std::min(double(0.99f), conic[g * 4 + 3] * std::exp(power));
+                            if (alpha < double(1.0f / 255.0f)) continue;
+                            const double tn = Tr * (1.0 - alpha);
+                            if (tn < double(0.0001f)) break;
+                            Tr = tn;
+                            blend.push_back(i);
+                        }
+                    }
+                    // FP64 forward over the blended list, then its exact adjoint
+                    const size_t L = blend.size();
+                    al.resize(L);
+                    Ts.resize(L + 1);
+                    Ts[0] = 1.0;
+                    double col[3] = {0, 0, 0};
+                    for (size_t k = 0; k < L; ++k) {
+                        const uint32_t g = vals[blend[k]];
+                        const double dx = xy[g * 2] - x, dy = xy[g * 2 + 1] - y;
+                        const double power = -0.5 * (conic[g * 4] * dx * dx + conic[g * 4 + 2] * dy * dy) -
+                                             conic[g * 4 + 1] * dx * dy;
+                        al[k] = std::min(double(0.99f), conic[g * 4 + 3] * std::exp(power));
+                        for (int ch = 0; ch < 3; ++ch) col[ch] += rgb[g * 4 + ch] * al[k] * Ts[k];
+                        Ts[k + 1] = Ts[k] * (1.0 - al[k]);
+                    }
+                    const size_t pix = (size_t)y * W + x;
+                    if (image)
+                        for (int ch = 0; ch < 3; ++ch) image[(size_t)ch * W * H + pix] = col[ch];
+                    const double gC[3] = {dLdC[pix], dLdC[(size_t)W * H + pix], dLdC[(size_t)2 * W * H + pix]};
+                    double Bk[3] = {0, 0, 0};  // colour behind splat k, normalised by T_{k+1}
+                    for (size_t kk = L; kk-- > 0;) {
+                        const uint32_t i = blend[kk];
+                        const uint32_t g = vals[i];
+                        double* a = &acc[(size_t)(i - start) * G2];
+                        double* am = &accm[(size_t)(i - start) * G2];
+                        const double alpha = al[kk], T = Ts[kk];
+                        double gal = 0;
+                        for (int ch = 0; ch < 3; ++ch) {
+                            a[6 + ch] += alpha * T * gC[ch];
+                            am[6 + ch] += std::fabs(alpha * T * gC[ch]);
+                            gal += (rgb[g * 4 + ch] - Bk[ch]) * gC[ch];
+                        }
+                        gal *= T;
+                        const double A = conic[g * 4], B = conic[g * 4 + 1], C = conic[g * 4 + 2];
+                        const double op = conic[g * 4 + 3];
+                        const double dx = xy[g * 2] - x, dy = xy[g * 2 + 1] - y;
+                        const double e = std::exp(-0.5 * (A * dx * dx + C * dy * dy) - B * dx * dy);
+                        if (op * e < double(0.99f)) {
+                            const double gpow = alpha * gal;
+                            const double t[6] = {gpow * (-(A * dx + B * dy)), gpow * (-(C * dy + B * dx)),
+                                                 gpow * (-0.5 * dx * dx), gpow * (-dx * dy), gpow * (-0.5 * dy * dy),
+                                                 e * gal};
+                            for (int q = 0; q < 6; ++q) {
+                                a[q] += t[q];
+                                am[q] += std::fabs(t[q]);
+                            }
+                        }
+                        for (int ch = 0; ch < 3; ++ch) Bk[ch] = alpha * rgb[g * 4 + ch] + (1 - alpha) * Bk[ch];
+                    }
+                }
+            for (uint32_t i = start; i < end; ++i) {
+                const double* a = &acc[(size_t)(i - start) * G2];
+                const double* am = &accm[(size_t)(i - start) * G2];
+                double* dst = &grad2d[(size_t)vals[i] * G2];
+                double* dm = grad2d_mass ? &grad2d_mass[(size_t)vals[i] * G2] : nullptr;
+                for (int k = 0; k < 9; ++k)
+                    if (a[k] != 0.0 || am[k] != 0.0) {
+#pragma omp atomic
+                        dst[k] += a[k];
+                        if (dm) {
+#pragma omp atomic
+                            dm[k] += am[k];
+                        }
+                    }
+            }
+        }
+    }
+    Packed P{packed, n};
+#pragma omp parallel for schedule(static)
+    for (int64_t g = 0; g < (int64_t)n; ++g)
+        preprocess_backward_one(P, (uint32_t)g, cam, &grad2d[g * G2], grad, N, (size_t)g);
+}
 }  // namespace
 
 extern "C" {
@@ -549,10 +915,92 @@ int orc_render_f32(const float* packed, uint32_t n, const void* cam, uint32_t ti
                          use_radix);
 }
 
+void orc_backward_f64(const float* packed, uint32_t n, const void* cam, uint32_t tile, const double* dLdC,
+                      int decisions_f32, double* grad, double* grad2d, double* image, double* grad2d_mass) {
+    backward_f64(packed, n, *(const Cam*)cam, tile, dLdC, decisions_f32, grad, grad2d, image, grad2d_mass);
+}
+
+// |J|^T m per splat: J the 59x9 Jacobian of the preprocess adjoint (columns
+// from unit 2-D gradients), m the absolute mass of the 2-D gradient terms.
+// Bounds the magnitude of the terms each parameter gradient sums.
+void orc_param_mass_f64(const float* packed, uint32_t n, const void* cam, const double* grad2d_mass, double* out) {
+    Packed P{packed, n};
+    const size_t N = n;
+#pragma omp parallel for schedule(static)
+        for (int64_t g = 0; g < (int64_t)n; ++g) {
+            double acc[59] = {0};
+            for (int k = 0; k < 9; ++k) {
+                const double mk = grad2d_mass[g * G2 + k];
+                if (mk == 0.0) continue;
+                double unit[G2] = {0};
+                unit[k] = 1.0;
+                double tmp[59];
+                preprocess_backward_one(P, (uint32_t)g, *(const Cam*)cam, unit, tmp, 1, 0);
+                for (int ch = 0; ch < 59; ++ch) acc[ch] += std::fabs(tmp[ch]) * mk;
+            }
+            for (int ch = 0; ch < 59; ++ch) out[(size_t)ch * N + g] = acc[ch];
+        }
+}
+
+void orc_preprocess_backward_f64(const float* packed, uint32_t n, const void* cam, const double* grad2d,
+                                 double* grad) {
+    Packed P{packed, n};
+    for (uint32_t g = 0; g < n; ++g)
+        preprocess_backward_one(P, g, *(const Cam*)cam, &grad2d[(size_t)g * G2], grad, n, g);
+}
+
+// Central differences of L = sum(dLdC * C) of the FP64 frame for parameter
+// `ch` (planar channel 0..58) of splats[k]: fd[k*59+ch], and same[k*59+ch] = 1
+// when both perturbed frames take the same discrete decisions.
+void orc_fd_f64(const float* packed, uint32_t n, const void* cam, uint32_t tile, const double* dLdC,
+                const uint32_t* splats, uint32_t count, float eps, double* fd, uint8_t* same) {
+    const Cam& C = *(const Cam*)cam;
+    const uint32_t W = (uint32_t)C.texture_size[0], H = (uint32_t)C.texture_size[1];
+    const size_t np = (size_t)n * 59;
+    // planar channel -> packed offset for splat g
+    auto offset = [n](int ch, uint32_t g) -> size_t {
+        const size_t N = n;
+        if (ch < 3) return (size_t)g * 3 + ch;
+        if (ch < 6) return N * 3 + (size_t)g * 3 + (ch - 3);
+        if (ch < 10) return N * 6 + (size_t)g * 4 + (ch - 6);
+        if (ch < 13) return N * 10 + (size_t)g * 3 + (ch - 10);
+        if (ch < 14) return N * 13 + g;
+        return N * 14 + (size_t)g * 45 + (ch - 14);
+    };
+#pragma omp parallel
+    {
+        std::vector<float> pk(packed, packed + np);
+        std::vector<double> img((size_t)3 * W * H), tf((size_t)W * H);
+        std::vector<uint32_t> last((size_t)W * H);
+#pragma omp for schedule(dynamic, 1)
+        for (int64_t job = 0; job < (int64_t)count * 59; ++job) {
+            const uint32_t g = splats[job / 59];
+            const int ch = (int)(job % 59);
+            const size_t o = offset(ch, g);
+            const float v = packed[o];
+            const float vp = v + eps, vm = v - eps;
+            double L[2];
+            uint64_t sig[2];
+            for (int side = 0; side < 2; ++side) {
+                pk[o] = side == 0 ? vp : vm;
+                uint64_t inst = 0;
+                render<double>(pk.data(), n, C, tile, 0, img.data(), tf.data(), last.data(), &inst, nullptr, 0,
+                               &sig[side]);
+                double acc = 0;
+                for (size_t i = 0; i < img.size(); ++i) acc += img[i] * dLdC[i];
+                L[side] = acc;
+            }
+            pk[o] = v;
+            fd[job] = (L[0] - L[1]) / ((double)vp - (double)vm);
+            same[job] = sig[0] == sig[1] ? 1 : 0;
+        }
+    }
+}
+
 int orc_render_f64(const float* packed, uint32_t n, const void* cam, uint32_t tile, uint64_t cap,
-                   double* out, double* tfinal, uint32_t* last, uint64_t* instances) {
+                   double* out, double* tfinal, uint32_t* last, uint64_t* instances, uint64_t* signature) {
     return render<double>(packed, n, *(const Cam*)cam, tile, cap, out, tfinal, last, instances, nullptr,
-                          0);
+                          0, signature);
 }
 
 }  // extern "C"

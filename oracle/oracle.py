@@ -56,7 +56,13 @@ def lib():
                                      _f32p, _f32p, _u32p, _u64p, _f64p, ctypes.c_int]
         L.orc_render_f64.restype = ctypes.c_int
         L.orc_render_f64.argtypes = [_f32p, ctypes.c_uint32, _f32p, ctypes.c_uint32, ctypes.c_uint64,
-                                     _f64p, _f64p, _u32p, _u64p]
+                                     _f64p, _f64p, _u32p, _u64p, ctypes.c_void_p]
+        L.orc_backward_f64.argtypes = [_f32p, ctypes.c_uint32, _f32p, ctypes.c_uint32, _f64p, ctypes.c_int,
+                                       _f64p, _f64p, ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_param_mass_f64.argtypes = [_f32p, ctypes.c_uint32, _f32p, _f64p, _f64p]
+        L.orc_fd_f64.argtypes = [_f32p, ctypes.c_uint32, _f32p, ctypes.c_uint32, _f64p, _u32p, ctypes.c_uint32,
+                                 ctypes.c_float, _f64p, np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")]
+        L.orc_preprocess_backward_f64.argtypes = [_f32p, ctypes.c_uint32, _f32p, _f64p, _f64p]
         _lib = L
     return _lib
 
@@ -131,8 +137,10 @@ def raster(width, height, tile, ranges, vals, conic, xy, rgb, fp64=False):
     return out.reshape(3, height, width), tf.reshape(height, width), last.reshape(height, width)
 
 
-def render(packed, n, uni, tile=16, cap=0, fp64=False, use_radix=False):
-    """Whole-frame oracle.  Returns (rgb[3,H,W], T_final[H,W], last[H,W], instances, stage_ms)."""
+def render(packed, n, uni, tile=16, cap=0, fp64=False, use_radix=False, signature=False):
+    """Whole-frame oracle.  Returns (rgb[3,H,W], T_final[H,W], last[H,W], instances, stage_ms);
+    with signature=True (FP64 only) the last item is instead the hash of every
+    discrete decision of the frame."""
     W, H = int(uni[40]), int(uni[41])
     dt = np.float64 if fp64 else np.float32
     out = np.zeros(3 * W * H, dt)
@@ -143,7 +151,51 @@ def render(packed, n, uni, tile=16, cap=0, fp64=False, use_radix=False):
     uni = np.ascontiguousarray(uni, np.float32)
     packed = np.ascontiguousarray(packed, np.float32)
     if fp64:
-        lib().orc_render_f64(packed, n, uni, tile, cap, out, tf, last, inst)
+        sig = np.zeros(1, np.uint64)
+        lib().orc_render_f64(packed, n, uni, tile, cap, out, tf, last, inst, sig.ctypes.data if signature else None)
+        if signature:
+            ms = int(sig[0])
     else:
         lib().orc_render_f32(packed, n, uni, tile, cap, out, tf, last, inst, ms, 1 if use_radix else 0)
     return (out.reshape(3, H, W), tf.reshape(H, W), last.reshape(H, W), int(inst[0]), ms)
+
+
+def backward(packed, n, uni, dLdC, tile=16, decisions_f32=True, want_image=False, want_mass=False):
+    """FP64 adjoint.  Returns (grad planar [59,N], grad2d [N,12], image [3,H,W] or None), plus
+    with want_mass=True (grad2d_mass [N,12], param_mass [59,N]): the summed absolute values of
+    the terms each gradient adds up (the scale its rounding error is relative to)."""
+    W, H = int(uni[40]), int(uni[41])
+    packed = np.ascontiguousarray(packed, np.float32)
+    uni = np.ascontiguousarray(uni, np.float32)
+    grad = np.zeros(59 * n, np.float64)
+    g2 = np.zeros(12 * n, np.float64)
+    img = np.zeros(3 * W * H, np.float64) if want_image else None
+    m2 = np.zeros(12 * n, np.float64) if want_mass else None
+    lib().orc_backward_f64(packed, n, uni, tile, np.ascontiguousarray(dLdC, np.float64).ravel(),
+                           1 if decisions_f32 else 0, grad, g2, img.ctypes.data if want_image else None,
+                           m2.ctypes.data if want_mass else None)
+    out = (grad.reshape(59, n), g2.reshape(n, 12), (img.reshape(3, H, W) if want_image else None))
+    if want_mass:
+        pm = np.zeros(59 * n, np.float64)
+        lib().orc_param_mass_f64(packed, n, uni, m2, pm)
+        out = out + (m2.reshape(n, 12), pm.reshape(59, n))
+    return out
+
+
+def preprocess_backward(packed, n, uni, grad2d):
+    grad = np.zeros(59 * n, np.float64)
+    lib().orc_preprocess_backward_f64(np.ascontiguousarray(packed, np.float32), n, np.ascontiguousarray(uni, np.float32),
+                                      np.ascontiguousarray(grad2d, np.float64).ravel(), grad)
+    return grad.reshape(59, n)
+
+
+def finite_differences(packed, n, uni, dLdC, splats, eps=1e-4, tile=16):
+    """FP64 central differences of sum(dLdC * image) for every parameter of the
+    given splats.  Returns (fd [59, k], same_branch [59, k] bool)."""
+    splats = np.ascontiguousarray(splats, np.uint32)
+    k = splats.size
+    fd = np.zeros(k * 59, np.float64)
+    same = np.zeros(k * 59, np.uint8)
+    lib().orc_fd_f64(np.ascontiguousarray(packed, np.float32), n, np.ascontiguousarray(uni, np.float32), tile,
+                     np.ascontiguousarray(dLdC, np.float64).ravel(), splats, k, eps, fd, same)
+    return fd.reshape(k, 59).T, same.reshape(k, 59).T.astype(bool)

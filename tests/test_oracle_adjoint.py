@@ -1,0 +1,71 @@
+"""Pins of the FP64 adjoint (oracle.backward) — CPU only.
+
+1. Central finite differences (eps 1e-4, FP64 forward) on 256 randomly chosen
+   splats, every one of the 59 parameters.  A component whose +eps and -eps
+   frames differ in any discrete decision (tile membership, alpha cutoff,
+   saturation stop, colour clamp) sits on a cliff of the renderer and is
+   excluded; the excluded share is bounded.
+2. The adjoint's forward image equals the FP64 forward.
+3. Decisions taken from FP32 give the same gradient as FP64 decisions where
+   the two frames agree (the branch choice is the only difference).
+"""
+import numpy as np
+
+from gsr import scenes
+
+
+def _scene():
+    sc = scenes.uniform_scene(384, seed=7, extent=1.0, scale_mu=-2.5, scale_sigma=0.4)
+    cam = scenes.Camera((0.3, 0.2, -4.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 50.0, 96, 80)
+    return sc, cam
+
+
+def test_adjoint_image_is_fp64_forward(orc):
+    sc, cam = _scene()
+    u = cam.uniform(sc.count)
+    dLdC = np.random.default_rng(0).normal(size=(3, 80, 96))
+    _, _, img = orc.backward(sc.packed(), sc.count, u, dLdC, decisions_f32=False, want_image=True)
+    ref, _, _, _, _ = orc.render(sc.packed(), sc.count, u, fp64=True)
+    assert np.abs(img - ref).max() < 1e-12
+
+
+def test_adjoint_matches_central_differences(orc):
+    sc, cam = _scene()
+    u = cam.uniform(sc.count)
+    n = sc.count
+    rng = np.random.default_rng(1)
+    dLdC = rng.normal(size=(3, cam.height, cam.width))
+    planar = sc.planar()
+    grad, _, _ = orc.backward(sc.packed(), n, u, dLdC, decisions_f32=False)
+    live = np.nonzero(np.abs(grad).sum(0) > 0)[0]
+    chosen = rng.choice(live, size=min(256, live.size), replace=False)
+    fd, same = orc.finite_differences(sc.packed(), n, u, dLdC, chosen, eps=1e-4)
+    an = grad[:, chosen]
+    errs = (np.abs(fd - an) / np.maximum(np.maximum(np.abs(an), np.abs(fd)), 1e-4))[same]
+    assert same.mean() > 0.9, same.mean()
+    assert np.quantile(errs, 0.999) < 1e-3 and errs.max() < 1e-2, (errs.max(), np.quantile(errs, 0.999))
+    # every parameter group is exercised on the same branch
+    for sl in (slice(0, 3), slice(3, 6), slice(6, 10), slice(10, 13), slice(13, 14), slice(14, 59)):
+        assert same[sl].sum() > 0.8 * same[sl].size
+
+
+def test_fp32_decisions_agree_with_fp64_decisions(orc):
+    sc, cam = _scene()
+    u = cam.uniform(sc.count)
+    dLdC = np.random.default_rng(4).normal(size=(3, 80, 96))
+    a, a2, _ = orc.backward(sc.packed(), sc.count, u, dLdC, decisions_f32=True)
+    b, b2, _ = orc.backward(sc.packed(), sc.count, u, dLdC, decisions_f32=False)
+    rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+    assert rel < 1e-2
+    same = np.isclose(a2, b2, rtol=1e-9, atol=1e-12).all(1)
+    assert same.mean() > 0.9
+
+
+def test_gradient_mass_bounds_the_gradient(orc):
+    sc, cam = _scene()
+    u = cam.uniform(sc.count)
+    dLdC = np.random.default_rng(5).normal(size=(3, 80, 96))
+    g, g2, _, m2, pm = orc.backward(sc.packed(), sc.count, u, dLdC, want_mass=True)
+    assert (m2[:, :9] >= np.abs(g2[:, :9]) * (1 - 1e-12)).all()
+    assert (pm >= np.abs(g) * (1 - 1e-9) - 1e-300).all()
+    assert (m2[:, :9] > 0).any(1).sum() > sc.count // 2

@@ -1,13 +1,454 @@
-// backward.hip — placeholder entry points filled in by the adjoint kernels.
+// backward.hip — adjoint of the forward frame (the reference is forward-only;
+// this differentiates src/shaders/raseter.wgsl:35-117 and
+// src/shaders/preprocess.wgsl:58-284).
+//
+// k_raster_backward: one wave64 per tile with the forward's pixel layout.
+//   Each pixel replays its splats back to front, starting at its stored last
+//   contributor with its stored final transmittance; alphas are recomputed
+//   with the forward's exact FP32 steps (so the blend decisions are the
+//   forward's), T is recovered as T / (1 - alpha), and the colour behind each
+//   splat is accumulated on the way.  Per splat the 9 gradient terms of the
+//   lane's pixels are summed in registers, reduced across the wave with DPP,
+//   and one lane adds them to the splat's 2-D gradient with float atomics.
+// k_preprocess_backward: one thread per splat, chains the 2-D gradient
+//   (pixel position, conic, opacity, colour) back to the 59 parameters.
+//
+// 2-D gradient layout per splat (12 floats): d/dx, d/dy of the pixel
+// position; d/dA, d/dB, d/dC of the conic; d/d opacity; d/d(r, g, b); pad.
+#include <cstring>
+
 #include "gsr_internal.h"
+#include "wave.h"
+
+namespace gsr {
+namespace {
+
+__device__ __forceinline__ float bcastf(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+// Full-wave sum; lane 63 holds the total (same DPP pattern as the scans).
+__device__ __forceinline__ float wave_sum_f32(float v) {
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xF, 0xF, false));
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x112, 0xF, 0xF, false));
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x114, 0xF, 0xF, false));
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x118, 0xF, 0xF, false));
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x142, 0xA, 0xF, false));
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x143, 0xC, 0xF, false));
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, m, 64));
+    return v;
+}
+
+template <int TS>
+__global__ __launch_bounds__(64) void k_raster_backward(
+    uint32_t W, uint32_t H, uint32_t tiles_x, uint32_t splats, const uint32_t* n_dev,
+    const uint2* __restrict__ ranges, const uint32_t* __restrict__ vals, const float4* __restrict__ conic,
+    const float2* __restrict__ xy, const float4* __restrict__ rgb, const float* __restrict__ tfinal,
+    const uint32_t* __restrict__ last, const float* __restrict__ dLdC, float* __restrict__ grad2d) {
+    constexpr int PPL = TS * TS / 64;
+    constexpr int STEP = 64 / TS;
+    const uint32_t tile = blockIdx.x;
+    const uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t x = tx * TS + lane % TS;
+    const uint32_t y0 = ty * TS + lane / TS;
+    const float fx = (float)x;
+    const size_t plane = (size_t)W * H;
+    float fy[PPL], T[PPL], gr[PPL], gg[PPL], gb[PPL], Br[PPL], Bg[PPL], Bb[PPL];
+    uint32_t lst[PPL];
+    uint32_t lane_max = 0;
+#pragma unroll
+    for (int k = 0; k < PPL; ++k) {
+        const uint32_t y = y0 + k * STEP;
+        fy[k] = (float)y;
+        Br[k] = Bg[k] = Bb[k] = 0.0f;
+        if (x < W && y < H) {
+            const size_t o = (size_t)y * W + x;
+            T[k] = tfinal[o];
+            lst[k] = last[o];
+            gr[k] = dLdC[o];
+            gg[k] = dLdC[plane + o];
+            gb[k] = dLdC[2 * plane + o];
+        } else {
+            T[k] = 1.0f;
+            lst[k] = 0;
+            gr[k] = gg[k] = gb[k] = 0.0f;
+        }
+        lane_max = max(lane_max, lst[k]);
+    }
+    const uint32_t maxlast = wave_max_u32(lane_max);
+    if (maxlast == 0) return;
+    const uint2 r = ranges[tile];
+    const uint32_t n_inst = *n_dev;
+    const uint32_t start = r.x;
+    const uint32_t stop = min(min(r.y, n_inst), start + maxlast);  // exclusive end of the replay
+    if (stop <= start) return;
+    for (int64_t b = (int64_t)start + (int64_t)((stop - start - 1) / 64) * 64; b >= (int64_t)start; b -= 64) {
+        const uint32_t idx = (uint32_t)b + lane;
+        float4 co = make_float4(0.f, 0.f, 0.f, 0.f), cl = make_float4(0.f, 0.f, 0.f, 0.f);
+        float2 p = make_float2(0.f, 0.f);
+        uint32_t gid = 0xFFFFFFFFu;
+        if (idx < stop) {
+            const uint32_t g = vals[idx];
+            if (g < splats) {
+                gid = g;
+                co = conic[g];
+                p = xy[g];
+                cl = rgb[g];
+            }
+        }
+        const int m = (int)min(64u, stop - (uint32_t)b);
+        for (int j = m - 1; j >= 0; --j) {
+            const uint32_t pos = (uint32_t)b - start + (uint32_t)j + 1;
+            const float gx = bcastf(p.x, j), gy = bcastf(p.y, j);
+            const float A = bcastf(co.x, j), B = bcastf(co.y, j), C = bcastf(co.z, j), op = bcastf(co.w, j);
+            const float cr = bcastf(cl.x, j), cg = bcastf(cl.y, j), cb = bcastf(cl.z, j);
+            const float dx = gx - fx;
+            const float ad = A * dx, bd = B * dx;
+            float a_x = 0.f, a_y = 0.f, a_A = 0.f, a_B = 0.f, a_C = 0.f, a_op = 0.f, a_r = 0.f, a_g = 0.f, a_b = 0.f;
+            bool any = false;
+#pragma unroll
+            for (int k = 0; k < PPL; ++k) {
+                if (pos > lst[k]) continue;
+                const float dy = gy - fy[k];
+                const float q = __builtin_fmaf(ad, dx, (C * dy) * dy);
+                const float power = __builtin_fmaf(-0.5f, q, -(bd * dy));
+                if (power > 0.0f || power < -5.55f) continue;
+                const float e = exp_core(power);
+                const float oe = op * e;
+                const float alpha = fminf(kAlphaClamp, oe);
+                if (alpha < kAlphaMin) continue;
+                // blended in the forward: undo it
+                const float Tb = T[k] / (1.0f - alpha);
+                T[k] = Tb;
+                const float gal = ((cr - Br[k]) * gr[k] + (cg - Bg[k]) * gg[k] + (cb - Bb[k]) * gb[k]) * Tb;
+                const float w = alpha * Tb;
+                a_r += w * gr[k];
+                a_g += w * gg[k];
+                a_b += w * gb[k];
+                if (oe <= kAlphaClamp) {
+                    const float gpow = alpha * gal;
+                    a_op += e * gal;
+                    a_A += gpow * (-0.5f * dx * dx);
+                    a_B += gpow * (-dx * dy);
+                    a_C += gpow * (-0.5f * dy * dy);
+                    a_x += gpow * (-(A * dx + B * dy));
+                    a_y += gpow * (-(C * dy + B * dx));
+                }
+                const float om = 1.0f - alpha;
+                Br[k] = alpha * cr + om * Br[k];
+                Bg[k] = alpha * cg + om * Bg[k];
+                Bb[k] = alpha * cb + om * Bb[k];
+                any = true;
+            }
+            if (ballot(any) == 0) continue;
+            a_x = wave_sum_f32(a_x);
+            a_y = wave_sum_f32(a_y);
+            a_A = wave_sum_f32(a_A);
+            a_B = wave_sum_f32(a_B);
+            a_C = wave_sum_f32(a_C);
+            a_op = wave_sum_f32(a_op);
+            a_r = wave_sum_f32(a_r);
+            a_g = wave_sum_f32(a_g);
+            a_b = wave_sum_f32(a_b);
+            const uint32_t g = (uint32_t)__builtin_amdgcn_readlane((int)gid, j);
+            if (lane == 63 && g < splats) {
+                float* dst = grad2d + (size_t)g * 12;
+                unsafeAtomicAdd(dst + 0, a_x);
+                unsafeAtomicAdd(dst + 1, a_y);
+                unsafeAtomicAdd(dst + 2, a_A);
+                unsafeAtomicAdd(dst + 3, a_B);
+                unsafeAtomicAdd(dst + 4, a_C);
+                unsafeAtomicAdd(dst + 5, a_op);
+                unsafeAtomicAdd(dst + 6, a_r);
+                unsafeAtomicAdd(dst + 7, a_g);
+                unsafeAtomicAdd(dst + 8, a_b);
+            }
+        }
+    }
+}
+
+constexpr float SH_C0 = 0.28209479177387814f;
+constexpr float SH_C1 = 0.4886025119029199f;
+constexpr float SH_C2_0 = 1.0925484305920792f, SH_C2_1 = -1.0925484305920792f,
+                SH_C2_2 = 0.31539156525252005f, SH_C2_3 = -1.0925484305920792f,
+                SH_C2_4 = 0.5462742152960396f;
+constexpr float SH_C3_0 = -0.5900435899266435f, SH_C3_1 = 2.890611442640554f,
+                SH_C3_2 = -0.4570457994644658f, SH_C3_3 = 0.3731763325901154f,
+                SH_C3_4 = -0.4570457994644658f, SH_C3_5 = 1.445305721320277f,
+                SH_C3_6 = -0.5900435899266435f;
+
+__global__ __launch_bounds__(256) void k_preprocess_backward(const float* __restrict__ P, uint32_t n, CamK cam,
+                                                             const float4* __restrict__ grad2d,
+                                                             float* __restrict__ grad) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const size_t N = n;
+    const float4 G0 = grad2d[(size_t)g * 3], G1 = grad2d[(size_t)g * 3 + 1], G2 = grad2d[(size_t)g * 3 + 2];
+    const float Gv[9] = {G0.x, G0.y, G0.z, G0.w, G1.x, G1.y, G1.z, G1.w, G2.x};
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) any |= Gv[k] != 0.0f;
+    const float px = P[CH_POS * N + g], py = P[(CH_POS + 1) * N + g], pz = P[(CH_POS + 2) * N + g];
+    const float* V = cam.view;
+    const float* PV = cam.view_proj;
+    const float tx = ((V[0] * px + V[4] * py) + V[8] * pz) + V[12];
+    const float ty = ((V[1] * px + V[5] * py) + V[9] * pz) + V[13];
+    const float tz = ((V[2] * px + V[6] * py) + V[10] * pz) + V[14];
+    if (!any || !(tz < -kNearView)) {
+        for (int ch = 0; ch < (int)kParams; ++ch) grad[(size_t)ch * N + g] = 0.0f;
+        return;
+    }
+    float gp0, gp1, gp2;
+    {   // pixel position <- ndc <- clip <- p
+        const float cx = ((PV[0] * px + PV[4] * py) + PV[8] * pz) + PV[12];
+        const float cy = ((PV[1] * px + PV[5] * py) + PV[9] * pz) + PV[13];
+        const float cw = ((PV[3] * px + PV[7] * py) + PV[11] * pz) + PV[15];
+        const float w = cw + kSmall;
+        const float gnx = Gv[0] * cam.texture_size[0] * 0.5f, gny = Gv[1] * cam.texture_size[1] * 0.5f;
+        const float iw = 1.0f / w;
+        const float gcx = gnx * iw, gcy = gny * iw, gcw = -(gnx * cx + gny * cy) * iw * iw;
+        gp0 = PV[0] * gcx + PV[1] * gcy + PV[3] * gcw;
+        gp1 = PV[4] * gcx + PV[5] * gcy + PV[7] * gcw;
+        gp2 = PV[8] * gcx + PV[9] * gcy + PV[11] * gcw;
+    }
+    // covariance chain
+    const float s0 = gsr_exp(P[(CH_SCALE + 0) * N + g]);
+    const float s1 = gsr_exp(P[(CH_SCALE + 1) * N + g]);
+    const float s2 = gsr_exp(P[(CH_SCALE + 2) * N + g]);
+    const float rw = P[(CH_ROT + 0) * N + g], rx = P[(CH_ROT + 1) * N + g];
+    const float ry = P[(CH_ROT + 2) * N + g], rz = P[(CH_ROT + 3) * N + g];
+    const float ql = sqrtf(((rx * rx + ry * ry) + rz * rz) + rw * rw);
+    const float qw = rw / ql, qx = rx / ql, qy = ry / ql, qz = rz / ql;
+    float R[3][3];
+    R[0][0] = 1.0f - 2.0f * (qy * qy + qz * qz); R[0][1] = 2.0f * (qx * qy - qw * qz); R[0][2] = 2.0f * (qx * qz + qw * qy);
+    R[1][0] = 2.0f * (qx * qy + qw * qz); R[1][1] = 1.0f - 2.0f * (qx * qx + qz * qz); R[1][2] = 2.0f * (qy * qz - qw * qx);
+    R[2][0] = 2.0f * (qx * qz - qw * qy); R[2][1] = 2.0f * (qy * qz + qw * qx); R[2][2] = 1.0f - 2.0f * (qx * qx + qy * qy);
+    const float sv[3] = {s0, s1, s2};
+    float M[3][3], S[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) M[r][c] = R[r][c] * sv[c];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) S[i][j] = M[i][0] * M[j][0] + M[i][1] * M[j][1] + M[i][2] * M[j][2];
+    const float limx = kFrustumGuard * cam.tan_fov[0], limy = kFrustumGuard * cam.tan_fov[1];
+    const float vx = tx / tz, vy = ty / tz;
+    const bool clx = vx < -limx || vx > limx, cly = vy < -limy || vy > limy;
+    const float ux = fminf(limx, fmaxf(-limx, vx)) * tz;
+    const float uy = fminf(limy, fmaxf(-limy, vy)) * tz;
+    const float uz = tz;
+    const float fx = cam.focal[0], fy = cam.focal[1];
+    const float iuz = 1.0f / uz, iuz2 = iuz * iuz;
+    const float J00 = fx * iuz, J02 = -fx * ux * iuz2, J11 = fy * iuz, J12 = -fy * uy * iuz2;
+    float Tm[2][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        Tm[0][c] = J00 * V[c * 4 + 0] + J02 * V[c * 4 + 2];
+        Tm[1][c] = J11 * V[c * 4 + 1] + J12 * V[c * 4 + 2];
+    }
+    float St0[3], St1[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        St0[i] = S[i][0] * Tm[0][0] + S[i][1] * Tm[0][1] + S[i][2] * Tm[0][2];
+        St1[i] = S[i][0] * Tm[1][0] + S[i][1] * Tm[1][1] + S[i][2] * Tm[1][2];
+    }
+    const float a = Tm[0][0] * St0[0] + Tm[0][1] * St0[1] + Tm[0][2] * St0[2] + 0.3f;
+    const float b = Tm[0][0] * St1[0] + Tm[0][1] * St1[1] + Tm[0][2] * St1[2];
+    const float c = Tm[1][0] * St1[0] + Tm[1][1] * St1[1] + Tm[1][2] * St1[2] + 0.3f;
+    const float det = a * c - b * b;
+    const float id2 = 1.0f / (det * det);
+    const float gA = Gv[2], gB = Gv[3], gC = Gv[4];
+    const float ga = (gA * (-c * c) + gB * (b * c) + gC * (-b * b)) * id2;
+    const float gbb = (gA * (2.0f * b * c) - gB * (det + 2.0f * b * b) + gC * (2.0f * a * b)) * id2;
+    const float gc = (gA * (-b * b) + gB * (a * b) + gC * (-a * a)) * id2;
+    float gt0[3], gt1[3], gS[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        gt0[i] = 2.0f * ga * St0[i] + gbb * St1[i];
+        gt1[i] = gbb * St0[i] + 2.0f * gc * St1[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            gS[i][j] = ga * Tm[0][i] * Tm[0][j] + gbb * Tm[0][i] * Tm[1][j] + gc * Tm[1][i] * Tm[1][j];
+    float gJ00 = 0.f, gJ02 = 0.f, gJ11 = 0.f, gJ12 = 0.f;
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc) {
+        gJ00 += gt0[cc] * V[cc * 4 + 0];
+        gJ02 += gt0[cc] * V[cc * 4 + 2];
+        gJ11 += gt1[cc] * V[cc * 4 + 1];
+        gJ12 += gt1[cc] * V[cc * 4 + 2];
+    }
+    const float gux = gJ02 * (-fx * iuz2), guy = gJ12 * (-fy * iuz2);
+    const float guz = gJ00 * (-fx * iuz2) + gJ02 * (2.0f * fx * ux * iuz2 * iuz) + gJ11 * (-fy * iuz2) +
+                      gJ12 * (2.0f * fy * uy * iuz2 * iuz);
+    float gtx = clx ? 0.0f : gux, gty = cly ? 0.0f : guy, gtz = guz;
+    if (clx) gtz += gux * (vx > 0.0f ? limx : -limx);
+    if (cly) gtz += guy * (vy > 0.0f ? limy : -limy);
+    gp0 += V[0] * gtx + V[1] * gty + V[2] * gtz;
+    gp1 += V[4] * gtx + V[5] * gty + V[6] * gtz;
+    gp2 += V[8] * gtx + V[9] * gty + V[10] * gtz;
+    float gM[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc)
+            gM[r][cc] = (gS[r][0] + gS[0][r]) * M[0][cc] + (gS[r][1] + gS[1][r]) * M[1][cc] +
+                        (gS[r][2] + gS[2][r]) * M[2][cc];
+    float gR[3][3];
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc) {
+        float gsc = 0.f;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            gR[r][cc] = gM[r][cc] * sv[cc];
+            gsc += gM[r][cc] * R[r][cc];
+        }
+        grad[(size_t)(CH_SCALE + cc) * N + g] = gsc * sv[cc];
+    }
+    const float gqw = 2.0f * (-qz * gR[0][1] + qy * gR[0][2] + qz * gR[1][0] - qx * gR[1][2] - qy * gR[2][0] + qx * gR[2][1]);
+    const float gqx = 2.0f * (qy * gR[0][1] + qz * gR[0][2] + qy * gR[1][0] - 2.0f * qx * gR[1][1] - qw * gR[1][2] +
+                              qz * gR[2][0] + qw * gR[2][1] - 2.0f * qx * gR[2][2]);
+    const float gqy = 2.0f * (-2.0f * qy * gR[0][0] + qx * gR[0][1] + qw * gR[0][2] + qx * gR[1][0] + qz * gR[1][2] -
+                              qw * gR[2][0] + qz * gR[2][1] - 2.0f * qy * gR[2][2]);
+    const float gqz = 2.0f * (-2.0f * qz * gR[0][0] - qw * gR[0][1] + qx * gR[0][2] + qw * gR[1][0] - 2.0f * qz * gR[1][1] +
+                              qy * gR[1][2] + qx * gR[2][0] + qy * gR[2][1]);
+    const float dotq = qw * gqw + qx * gqx + qy * gqy + qz * gqz;
+    const float iql = 1.0f / ql;
+    grad[(size_t)(CH_ROT + 0) * N + g] = (gqw - qw * dotq) * iql;
+    grad[(size_t)(CH_ROT + 1) * N + g] = (gqx - qx * dotq) * iql;
+    grad[(size_t)(CH_ROT + 2) * N + g] = (gqy - qy * dotq) * iql;
+    grad[(size_t)(CH_ROT + 3) * N + g] = (gqz - qz * dotq) * iql;
+    // opacity
+    const float op = 1.0f / (1.0f + gsr_exp(-P[CH_OPACITY * N + g]));
+    grad[(size_t)CH_OPACITY * N + g] = Gv[5] * op * (1.0f - op);
+    // colour
+    const float vx3 = px - cam.eye[0], vy3 = py - cam.eye[1], vz3 = pz - cam.eye[2];
+    const float vl = sqrtf(vx3 * vx3 + vy3 * vy3 + vz3 * vz3);
+    const float X = vx3 / vl, Y = vy3 / vl, Z = vz3 / vl;
+    const float XX = X * X, YY = Y * Y, ZZ = Z * Z;
+    const float basis[15] = {-SH_C1 * Y, SH_C1 * Z, -SH_C1 * X, SH_C2_0 * X * Y, SH_C2_1 * Y * Z,
+                             SH_C2_2 * (2.0f * ZZ - XX - YY), SH_C2_3 * X * Z, SH_C2_4 * (XX - YY),
+                             SH_C3_0 * Y * (3.0f * XX - YY), SH_C3_1 * X * Y * Z, SH_C3_2 * Y * (4.0f * ZZ - XX - YY),
+                             SH_C3_3 * Z * (2.0f * ZZ - 3.0f * XX - 3.0f * YY), SH_C3_4 * X * (4.0f * ZZ - XX - YY),
+                             SH_C3_5 * Z * (XX - YY), SH_C3_6 * X * (XX - 3.0f * YY)};
+    const float dbx[15] = {0.f, 0.f, -SH_C1, SH_C2_0 * Y, 0.f, SH_C2_2 * (-2.0f * X), SH_C2_3 * Z, SH_C2_4 * 2.0f * X,
+                           SH_C3_0 * Y * 6.0f * X, SH_C3_1 * Y * Z, SH_C3_2 * Y * (-2.0f * X), SH_C3_3 * Z * (-6.0f * X),
+                           SH_C3_4 * (4.0f * ZZ - 3.0f * XX - YY), SH_C3_5 * Z * 2.0f * X, SH_C3_6 * (3.0f * XX - 3.0f * YY)};
+    const float dby[15] = {-SH_C1, 0.f, 0.f, SH_C2_0 * X, SH_C2_1 * Z, SH_C2_2 * (-2.0f * Y), 0.f, SH_C2_4 * (-2.0f * Y),
+                           SH_C3_0 * (3.0f * XX - 3.0f * YY), SH_C3_1 * X * Z, SH_C3_2 * (4.0f * ZZ - XX - 3.0f * YY),
+                           SH_C3_3 * Z * (-6.0f * Y), SH_C3_4 * X * (-2.0f * Y), SH_C3_5 * Z * (-2.0f * Y),
+                           SH_C3_6 * X * (-6.0f * Y)};
+    const float dbz[15] = {0.f, SH_C1, 0.f, 0.f, SH_C2_1 * Y, SH_C2_2 * 4.0f * Z, SH_C2_3 * X, 0.f, 0.f,
+                           SH_C3_1 * X * Y, SH_C3_2 * Y * 8.0f * Z, SH_C3_3 * (6.0f * ZZ - 3.0f * XX - 3.0f * YY),
+                           SH_C3_4 * X * 8.0f * Z, SH_C3_5 * (XX - YY), 0.f};
+    float gdx = 0.f, gdy = 0.f, gdz = 0.f;
+    const float gcol[3] = {Gv[6], Gv[7], Gv[8]};
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+        float acc = SH_C0 * P[(CH_DC + ch) * N + g];
+        float f[15];
+#pragma unroll
+        for (int k = 0; k < 15; ++k) {
+            f[k] = P[(CH_REST + 15 * ch + k) * N + g];
+            acc += basis[k] * f[k];
+        }
+        const float gch = (acc + 0.5f < 0.0f) ? 0.0f : gcol[ch];
+        grad[(size_t)(CH_DC + ch) * N + g] = SH_C0 * gch;
+#pragma unroll
+        for (int k = 0; k < 15; ++k) {
+            grad[(size_t)(CH_REST + 15 * ch + k) * N + g] = basis[k] * gch;
+            gdx += gch * dbx[k] * f[k];
+            gdy += gch * dby[k] * f[k];
+            gdz += gch * dbz[k] * f[k];
+        }
+    }
+    const float dd = X * gdx + Y * gdy + Z * gdz;
+    const float ivl = 1.0f / vl;
+    gp0 += (gdx - X * dd) * ivl;
+    gp1 += (gdy - Y * dd) * ivl;
+    gp2 += (gdz - Z * dd) * ivl;
+    grad[(size_t)(CH_POS + 0) * N + g] = gp0;
+    grad[(size_t)(CH_POS + 1) * N + g] = gp1;
+    grad[(size_t)(CH_POS + 2) * N + g] = gp2;
+}
+
+}  // namespace
+
+int launch_backward(Context* c, const float* params, uint32_t count, const CamK& cam, const float* dLdC,
+                    float* grad, float* grad2d_out) {
+    const uint32_t W = c->state_w, H = c->state_h, tile = c->state_tile;
+    const uint32_t tx = ceil_div(W, tile), ty = ceil_div(H, tile);
+    const size_t N = count;
+    float* g2 = (float*)buffer(c, "grad2d", N * 48);
+    if (!g2) return GSR_ERR_ALLOC;
+    int rc = launch_fill_u32(c, (uint32_t*)g2, 0u, N * 12);
+    if (rc) return rc;
+    const uint32_t* vals = (const uint32_t*)buffer(c, c->state_sorted_index ? "vals1" : "vals0", 0);
+    const uint2* ranges = (const uint2*)buffer(c, "ranges", 0);
+    const float4* conic = (const float4*)buffer(c, "conic", 0);
+    const float2* xy = (const float2*)buffer(c, "xy", 0);
+    const float4* rgb = (const float4*)buffer(c, "rgb", 0);
+    const float* tfinal = (const float*)buffer(c, "tfinal", 0);
+    const uint32_t* last = (const uint32_t*)buffer(c, "last", 0);
+    const FrameCounters* counters = (const FrameCounters*)buffer(c, "counters", 0);
+    mark(c, "backward raster");
+    const uint32_t tiles = tx * ty;
+    switch (tile) {
+        case 8:
+            hipLaunchKernelGGL(k_raster_backward<8>, dim3(tiles), dim3(64), 0, c->stream, W, H, tx, count,
+                               &counters->n, ranges, vals, conic, xy, rgb, tfinal, last, dLdC, g2);
+            break;
+        case 16:
+            hipLaunchKernelGGL(k_raster_backward<16>, dim3(tiles), dim3(64), 0, c->stream, W, H, tx, count,
+                               &counters->n, ranges, vals, conic, xy, rgb, tfinal, last, dLdC, g2);
+            break;
+        case 32:
+            hipLaunchKernelGGL(k_raster_backward<32>, dim3(tiles), dim3(64), 0, c->stream, W, H, tx, count,
+                               &counters->n, ranges, vals, conic, xy, rgb, tfinal, last, dLdC, g2);
+            break;
+        default:
+            return fail(c, GSR_ERR_STATE, "bad saved tile size %u", tile);
+    }
+    GSR_HIP(c, hipGetLastError());
+    mark(c, "backward preprocess");
+    hipLaunchKernelGGL(k_preprocess_backward, dim3(ceil_div(count, 256)), dim3(256), 0, c->stream, params, count, cam,
+                       (const float4*)g2, grad);
+    GSR_HIP(c, hipGetLastError());
+    mark_end(c, false);
+    if (grad2d_out) GSR_HIP(c, hipMemcpyAsync(grad2d_out, g2, N * 48, hipMemcpyDeviceToDevice, c->stream));
+    return GSR_OK;
+}
+
+}  // namespace gsr
 
 using namespace gsr;
 
 extern "C" {
 
-int gsr_backward(gsr_context* ctx, const float*, uint32_t, const gsr_camera*, const float*, float*, float*) {
+int gsr_backward(gsr_context* ctx, const float* params_dev, uint32_t count, const gsr_camera* cam,
+                 const float* dL_drgb_dev, float* grad_params_dev, float* grad2d_dev) {
     Context* c = reinterpret_cast<Context*>(ctx);
-    return fail(c, GSR_ERR_STATE, "gsr_backward: not available in this build");
+    if (!c) return fail(nullptr, GSR_ERR_ARGUMENT, "null context");
+    if (!params_dev || !cam || !dL_drgb_dev || !grad_params_dev)
+        return fail(c, GSR_ERR_ARGUMENT, "gsr_backward: null argument");
+    if (!c->have_state || c->state_count != count || c->state_w != (uint32_t)cam->texture_size[0] ||
+        c->state_h != (uint32_t)cam->texture_size[1])
+        return fail(c, GSR_ERR_STATE, "gsr_backward needs the matching gsr_forward with save_state = 1 first");
+    hipSetDevice(c->device);
+    CamK ck;
+    std::memcpy(&ck, cam, sizeof(ck));
+    ck.count = count;
+    return launch_backward(c, params_dev, count, ck, dL_drgb_dev, grad_params_dev, grad2d_dev);
 }
 
 int gsr_render_batch(gsr_context* ctx, const float*, uint32_t, const gsr_camera*, uint32_t, const gsr_options*, int,

@@ -61,9 +61,9 @@ void mark(Context* c, const char* label) {
     c->marks.emplace_back(label, e);
 }
 
-void mark_end(Context* c) {
+void mark_end(Context* c, bool frame_end) {
     if (!c->timing_on) return;
-    mark(c, "");
+    mark(c, frame_end ? "" : "\x01");
 }
 
 namespace {
@@ -76,7 +76,8 @@ void collect_timings(Context* c) {
     int frames = 0;
     for (size_t i = 0; i < c->marks.size(); ++i) {
         const std::string& label = c->marks[i].first;
-        if (label.empty()) { ++frames; continue; }
+        if (label.empty()) { ++frames; continue; }   // end of a frame
+        if (label[0] == '\x01') continue;            // end of a stage group inside a frame
         if (i + 1 >= c->marks.size()) break;
         float ms = 0.0f;
         hipEventElapsedTime(&ms, c->marks[i].second, c->marks[i + 1].second);
@@ -291,9 +292,14 @@ int gsr_to_rgba8(gsr_context* ctx, const float* rgb_dev, uint32_t width, uint32_
                  uint8_t* out_rgba_host) {
     Context* c = reinterpret_cast<Context*>(ctx);
     if (!c) return fail(nullptr, GSR_ERR_ARGUMENT, "null context");
-    if (!rgb_dev || !out_rgba_host || width == 0 || height == 0)
-        return fail(c, GSR_ERR_ARGUMENT, "gsr_to_rgba8: bad arguments");
+    if (!out_rgba_host || width == 0 || height == 0) return fail(c, GSR_ERR_ARGUMENT, "gsr_to_rgba8: bad arguments");
     const size_t px = (size_t)width * height;
+    if (!rgb_dev) {  // the context's last rendered frame
+        if (c->last_frame_w != width || c->last_frame_h != height)
+            return fail(c, GSR_ERR_STATE, "gsr_to_rgba8: no %ux%u frame was rendered last", width, height);
+        rgb_dev = (const float*)buffer(c, "image", px * 12);
+        if (!rgb_dev) return GSR_ERR_ALLOC;
+    }
     uint8_t* d = (uint8_t*)buffer(c, "rgba8", px * 4);
     if (!d) return GSR_ERR_ALLOC;
     hipLaunchKernelGGL(k_rgba8, dim3((uint32_t)((px + 255) / 256)), dim3(256), 0, c->stream, rgb_dev, width, height,

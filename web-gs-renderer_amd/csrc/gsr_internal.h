@@ -123,6 +123,7 @@ struct Context {
     uint32_t state_count = 0, state_w = 0, state_h = 0, state_tile = 16, state_key_bytes = 4;
     uint64_t state_capacity = 0;
     int state_sorted_index = 0;
+    uint32_t last_frame_w = 0, last_frame_h = 0;  // size of the frame in "image"
     // host pinned staging for small readbacks
     FrameCounters* host_counters = nullptr;
 };
@@ -141,7 +142,9 @@ int fail(Context* c, int code, const char* fmt, ...);
 
 // Timing marks on the context stream (no-ops unless timing is on).
 void mark(Context* c, const char* label);
-void mark_end(Context* c);
+// Closes the last stage; frame_end = false for a stage group (e.g. the
+// backward) that belongs to the same frame as the preceding forward.
+void mark_end(Context* c, bool frame_end = true);
 // After the stream has drained: turn the recorded frames' marks into timing
 // rows (per-frame averages over every frame since the last collection).
 void finish_frame_timings(Context* c);

@@ -99,6 +99,10 @@ int enqueue_forward(Context* c, const float* params, uint32_t count, const gsr_c
                              last)))
         return rc;
     mark_end(c);
+    if (!image_override) {
+        c->last_frame_w = p.W;
+        c->last_frame_h = p.H;
+    }
     c->have_state = save_state;
     c->state_count = count;
     c->state_w = p.W;

@@ -67,7 +67,7 @@ def build_addon() -> str | None:
     if not (os.path.exists(src) and os.path.exists(os.path.join(inc, "node_api.h"))):
         return None
     tmp = ADDON + ".tmp"
-    _run(["gcc", "-O2", "-std=c11", "-fPIC", "-shared", "-Wall", f"-I{inc}",
+    _run(["gcc", "-O2", "-std=c11", "-fPIC", "-shared", "-Wall", "-DNODE_GYP_MODULE_NAME=gsr_napi", f"-I{inc}",
           f"-I{os.path.join(os.path.dirname(PKG_DIR), 'include')}", src, "-o", tmp,
           f"-L{BUILD}", "-lgsr", "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, ADDON)
