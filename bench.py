@@ -69,6 +69,50 @@ def cpu_baseline(sc, u, budget_s: float = 15.0):
                       f"C++ pass graph (OpenMP, {threads} threads, stable LSD radix sort)"}
 
 
+def batch_leg(ctx, torch, lib, scenes, reps=10):
+    """64 cameras of mixed resolution over 500,000 clustered splats: the batch
+    (autotuned, hipGraph replays) against the plain path on the same cameras."""
+    log("batch: building the 500k-splat scene and 64 cameras")
+    sc, cams = scenes.workload_batch64()
+    us = [c.uniform(sc.count) for c in cams]
+    params = torch.from_numpy(sc.planar()).cuda()
+    n = sc.count
+    offs = ctx.batch_layout(n, us, lib.options(tile_size=8))
+    out = torch.empty(offs[-1], dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    log("batch: autotuning")
+    opts, tuned_ms = ctx.autotune_batch(params, n, us, lib.options(), out)
+    offs = ctx.batch_layout(n, us, opts)
+    out = torch.empty(offs[-1], dtype=torch.float32, device="cuda")
+    for _ in range(2):                       # eager frame + capture, then a replay
+        ctx.render_batch(params, n, us, opts, out, use_graph=True)
+    instances = ctx.batch_instance_count()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.render_batch(params, n, us, opts, out, use_graph=True)
+    t_batch = (time.perf_counter() - t0) / reps
+    # plain single-view path on the same cameras, back-to-back frames per camera
+    plain = []
+    popts = lib.options(tile_size=16)
+    for u in us:
+        W, H = int(u[40]), int(u[41])
+        img = torch.empty(3 * W * H, dtype=torch.float32, device="cuda")
+        ctx.forward(params, n, u, popts, img)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(5):
+            ctx.forward(params, n, u, popts, img)
+        ctx.synchronize()
+        plain.append((time.perf_counter() - t0) / 5)
+    t_plain = sum(plain) / len(plain)
+    ratio = (len(us) / t_batch) / (1.0 / t_plain)
+    return {"views": len(us), "splats": n, "instances": instances, "tile": opts.tile_size,
+            "raster_waves_per_cu": opts.raster_waves_per_cu, "sort_blocks_per_cu": opts.sort_blocks_per_cu,
+            "batch_ms": round(t_batch * 1e3, 4), "views_per_s": round(len(us) / t_batch, 2),
+            "plain_mean_ms_per_view": round(t_plain * 1e3, 4), "per_view_speedup_vs_plain": round(ratio, 3),
+            "autotune_best_ms": round(tuned_ms, 4)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -177,6 +221,8 @@ def main():
     extra["fwd_bwd_ms_per_step"] = round(dtb * 1000.0 / bsteps, 5)
     extra["fwd_bwd_stage_ms"] = {k: round(v, 5) for k, v in bw_ms.items()}
     del dldc, grad
+    ctx.scene_free("bench")
+    extra["batch"] = batch_leg(ctx, torch, lib, scenes)
     cpu = None
     if not args.no_cpu:
         log("cpu baseline")

@@ -65,6 +65,8 @@ typedef struct gsr_options {
     uint64_t instance_capacity;/* tile-instance slots; 0 = 64 * splat count (renderer.js:16,536) */
     uint32_t timing;           /* 1 = record hipEvents around every stage (gsr_get_timings)      */
     uint32_t save_state;       /* 1 = keep per-pixel T_final / last contributor for backward     */
+    uint32_t raster_waves_per_cu; /* batches: persistent raster waves per CU; 0 = 16              */
+    uint32_t sort_blocks_per_cu;  /* onesweep blocks per CU; 0 = 2                                 */
 } gsr_options;
 
 /* ---- timings --------------------------------------------------------- */
@@ -194,15 +196,36 @@ int gsr_backward(gsr_context* ctx, const float* params_dev, uint32_t count,
 /* ---- multi-view batch (no reference counterpart: the reference renders one view) ---- */
 /*
  * Renders `views` cameras of the same parameters in one launch sequence with
- * keys view|tile|depth (one sort serves all views) and a persistent (view, tile)
- * work queue.  out_rgb_dev receives the views back to back, each planar
+ * keys view|tile|depth (one sort serves all views) and a persistent (view,
+ * tile) work queue.  out_rgb_dev receives the views back to back, each planar
  * [3][H_v][W_v]; out_offsets (host, views+1 entries, may be NULL) receives the
- * float offset of each view's image.  use_graph = 1 captures the frame into a
- * hipGraph on first use and replays it for identical batch shapes.
+ * float offset of each view's image and the total.  out_rgb_dev == NULL only
+ * fills out_offsets.  use_graph = 1 captures the frame as a hipGraph after the
+ * first eager run of a batch shape and replays it while the shape (splat
+ * count, parameter and output pointers, view resolutions, options) is
+ * unchanged; cameras may change between replays.  When tile instances
+ * overflow the capacity (opts->instance_capacity, else a planner that starts
+ * at 16 per splat-view and grows) the frame is re-run once with room for the
+ * measured count.  Blocks until the frame is done.
  */
 int gsr_render_batch(gsr_context* ctx, const float* params_dev, uint32_t count,
                      const gsr_camera* cams, uint32_t views, const gsr_options* opts,
                      int use_graph, float* out_rgb_dev, uint64_t* out_offsets);
+
+/* Tile instances of the last batch frame (raw scan total over all views). */
+int gsr_batch_instance_count(gsr_context* ctx, uint64_t* instances);
+
+/*
+ * Autotuner for a batch shape: times tile size {8, 16, 32} x persistent raster
+ * waves per CU {8, 16, 32} x onesweep blocks per CU {2, 3} (three timed
+ * frames each, HIP events) and writes the fastest into *inout (other fields
+ * kept).  out_rgb_dev is used as the scratch output.  The tile size changes
+ * which splats reach a pixel's tile at the fringe of their 3-sigma square, so
+ * images are comparable only between renders with the same tile size.
+ */
+int gsr_autotune_batch(gsr_context* ctx, const float* params_dev, uint32_t count,
+                       const gsr_camera* cams, uint32_t views, gsr_options* inout,
+                       float* out_rgb_dev, double* best_ms);
 
 #ifdef __cplusplus
 }

@@ -97,6 +97,7 @@ constexpr double SH_C3[7] = {-0.5900435899266435, 2.890611442640554, -0.45704579
                              -0.5900435899266435};
 
 template <typename T> struct Pre {
+    uint8_t flags;   // bit 0-2: colour channel clamped at 0; bit 3/4: x/y frustum-guard clamp active
     uint32_t tiles;
     uint32_t meta[4];
     T conic[4];
@@ -172,6 +173,8 @@ void preprocess_one(const Packed& P, uint32_t g, const Cam& cam, uint32_t tile, 
     const T ux = std::min(limx, std::max(-limx, tx / tz)) * tz;
     const T uy = std::min(limy, std::max(-limy, ty / tz)) * tz;
     const T uz = tz;
+    if (tx / tz < -limx || tx / tz > limx) o.flags |= 8;
+    if (ty / tz < -limy || ty / tz > limy) o.flags |= 16;
     const T fx = cam.focal[0], fy = cam.focal[1];
     const T uz2 = uz * uz;
     const T J00 = fx / uz, J02 = (-fx * ux) / uz2;
@@ -227,6 +230,7 @@ void preprocess_one(const Packed& P, uint32_t g, const Cam& cam, uint32_t tile, 
         T acc = T(float(SH_C0)) * T(P.dc(g, ch));
         for (int k = 0; k < 15; ++k) acc = M<T>::fma(basis[k], T(P.rest(g, k, ch)), acc);
         acc = acc + T(0.5);
+        if (acc < T(0)) o.flags |= (uint8_t)(1u << ch);
         o.rgb[ch] = std::max(acc, T(0));
     }
     // tile rect, depth16, tiles touched (:122-143)
@@ -249,7 +253,7 @@ void preprocess_one(const Packed& P, uint32_t g, const Cam& cam, uint32_t tile, 
 
 template <typename T>
 void preprocess_all(const float* packed, uint32_t n, const Cam& cam, uint32_t tile,
-                    uint32_t* tiles, uint32_t* meta, T* conic, T* xy, T* rgb) {
+                    uint32_t* tiles, uint32_t* meta, T* conic, T* xy, T* rgb, uint8_t* flags = nullptr) {
     Packed P{packed, n};
 #pragma omp parallel for schedule(static)
     for (int64_t g = 0; g < (int64_t)n; ++g) {
@@ -261,6 +265,7 @@ void preprocess_all(const float* packed, uint32_t n, const Cam& cam, uint32_t ti
         xy[g * 2] = o.xy[0];
         xy[g * 2 + 1] = o.xy[1];
         for (int k = 0; k < 4; ++k) rgb[g * 4 + k] = o.rgb[k];
+        if (flags) flags[g] = o.flags;
     }
 }
 
@@ -515,8 +520,10 @@ constexpr int G2 = 12;
 
 // Per-splat preprocess adjoint (chain of preprocess.wgsl:80-135 in FP64).
 // Writes d/d(parameter ch) to out[ch * stride + index].
+// flags (may be null): the FP32 forward's clamp decisions for this splat
+// (see Pre::flags); without them the FP64 values decide.
 void preprocess_backward_one(const Packed& P, uint32_t g, const Cam& cam, const double* G, double* out,
-                             size_t stride, size_t index) {
+                             size_t stride, size_t index, const uint8_t* flags = nullptr) {
     auto put = [&](int ch, double v) { out[(size_t)ch * stride + index] = v; };
     const double px = P.pos(g, 0), py = P.pos(g, 1), pz = P.pos(g, 2);
     const float* V = cam.view;
@@ -557,9 +564,10 @@ void preprocess_backward_one(const Packed& P, uint32_t g, const Cam& cam, const 
         for (int j = 0; j < 3; ++j) S[i][j] = Mm[i][0] * Mm[j][0] + Mm[i][1] * Mm[j][1] + Mm[i][2] * Mm[j][2];
     const double limx = double(1.3f) * cam.tan_fov[0], limy = double(1.3f) * cam.tan_fov[1];
     const double vx = tx / tz, vy = ty / tz;
-    const bool clx = vx < -limx || vx > limx, cly = vy < -limy || vy > limy;
-    const double ux = std::min(limx, std::max(-limx, vx)) * tz;
-    const double uy = std::min(limy, std::max(-limy, vy)) * tz;
+    const bool clx = flags ? (*flags & 8) != 0 : (vx < -limx || vx > limx);
+    const bool cly = flags ? (*flags & 16) != 0 : (vy < -limy || vy > limy);
+    const double ux = (clx ? (vx > 0 ? limx : -limx) : vx) * tz;
+    const double uy = (cly ? (vy > 0 ? limy : -limy) : vy) * tz;
     const double uz = tz;
     const double fx = cam.focal[0], fy = cam.focal[1];
     const double J00 = fx / uz, J02 = -fx * ux / (uz * uz), J11 = fy / uz, J12 = -fy * uy / (uz * uz);
@@ -665,7 +673,8 @@ void preprocess_backward_one(const Packed& P, uint32_t g, const Cam& cam, const 
     for (int ch = 0; ch < 3; ++ch) {
         double acc = C0 * P.dc(g, ch);
         for (int k = 0; k < 15; ++k) acc += basis[k] * P.rest(g, k, ch);
-        const double gch = (acc + 0.5 < 0.0) ? 0.0 : G[6 + ch];
+        const bool clamped = flags ? ((*flags >> ch) & 1) != 0 : (acc + 0.5 < 0.0);
+        const double gch = clamped ? 0.0 : G[6 + ch];
         put(10 + ch, C0 * gch);
         for (int k = 0; k < 15; ++k) {
             const double f = P.rest(g, k, ch);
@@ -695,12 +704,13 @@ void backward_f64(const float* packed, uint32_t n, const Cam& cam, uint32_t tile
     std::vector<uint32_t> tiles(N), meta(N * 4);
     std::vector<double> conic(N * 4), xy(N * 2), rgb(N * 4);
     std::vector<float> conicf(N * 4), xyf(N * 2), rgbf(N * 4);
+    std::vector<uint8_t> flags32(decisions_f32 ? N : 0);
     {
         std::vector<uint32_t> t64(N), m64(N * 4);
         preprocess_all<double>(packed, n, cam, tile, t64.data(), m64.data(), conic.data(), xy.data(), rgb.data());
         if (decisions_f32) {
             preprocess_all<float>(packed, n, cam, tile, tiles.data(), meta.data(), conicf.data(), xyf.data(),
-                                  rgbf.data());
+                                  rgbf.data(), flags32.data());
         } else {
             tiles = t64;
             meta = m64;
@@ -725,6 +735,7 @@ void backward_f64(const float* packed, uint32_t n, const Cam& cam, uint32_t tile
     {
         std::vector<double> acc, accm;
         std::vector<uint32_t> blend;
+        std::vector<uint8_t> clampv;
         std::vector<double> al, Ts;
 #pragma omp for schedule(dynamic, 1)
         for (int64_t t = 0; t < ntile; ++t) {
@@ -739,6 +750,7 @@ void backward_f64(const float* packed, uint32_t n, const Cam& cam, uint32_t tile
                     if (x >= W || y >= H) continue;
                     // discrete decisions: which range positions this pixel blends
                     blend.clear();
+                    clampv.clear();
                     if (decisions_f32) {
                         float Tr = 1.0f;
                         for (uint32_t i = start; i < end; ++i) {
@@ -748,12 +760,14 @@ void backward_f64(const float* packed, uint32_t n, const Cam& cam, uint32_t tile
                             const float q = fmaf(A * dx, dx, (C * dy) * dy);
                             const float power = fmaf(-0.5f, q, -((B * dx) * dy));
                             if (power > 0.0f) continue;
-                            const float alpha = std::min(0.99f, conicf[g * 4 + 3] * exp_f32(power));
+                            const float oe = conicf[g * 4 + 3] * exp_f32(power);
+                            const float alpha = std::min(0.99f, oe);
                             if (alpha < 1.0f / 255.0f) continue;
                             const float tn = Tr * (1.0f - alpha);
                             if (tn < 0.0001f) break;
                             Tr = tn;
                             blend.push_back(i);
+                            clampv.push_back(oe > 0.99f ? 1 : 0);
                         }
                     } else {
                         double Tr = 1.0;
@@ -763,12 +777,14 @@ void backward_f64(const float* packed, uint32_t n, const Cam& cam, uint32_t tile
                             const double power = -0.5 * (conic[g * 4] * dx * dx + conic[g * 4 + 2] * dy * dy) -
                                                  conic[g * 4 + 1] * dx * dy;
                             if (power > 0.0) continue;
-                            const double alpha = std::min(double(0.99f), conic[g * 4 + 3] * std::exp(power));
+                            const double oe = conic[g * 4 + 3] * std::exp(power);
+                            const double alpha = std::min(double(0.99f), oe);
                             if (alpha < double(1.0f / 255.0f)) continue;
                             const double tn = Tr * (1.0 - alpha);
                             if (tn < double(0.0001f)) break;
                             Tr = tn;
                             blend.push_back(i);
+                            clampv.push_back(oe > double(0.99f) ? 1 : 0);
                         }
                     }
                     // FP64 forward over the blended list, then its exact adjoint
@@ -782,7 +798,7 @@ void backward_f64(const float* packed, uint32_t n, const Cam& cam, uint32_t tile
                         const double dx = xy[g * 2] - x, dy = xy[g * 2 + 1] - y;
                         const double power = -0.5 * (conic[g * 4] * dx * dx + conic[g * 4 + 2] * dy * dy) -
                                              conic[g * 4 + 1] * dx * dy;
-                        al[k] = std::min(double(0.99f), conic[g * 4 + 3] * std::exp(power));
+                        al[k] = clampv[k] ? double(0.99f) : conic[g * 4 + 3] * std::exp(power);
                         for (int ch = 0; ch < 3; ++ch) col[ch] += rgb[g * 4 + ch] * al[k] * Ts[k];
                         Ts[k + 1] = Ts[k] * (1.0 - al[k]);
                     }
@@ -797,25 +813,32 @@ void backward_f64(const float* packed, uint32_t n, const Cam& cam, uint32_t tile
                         double* a = &acc[(size_t)(i - start) * G2];
                         double* am = &accm[(size_t)(i - start) * G2];
                         const double alpha = al[kk], T = Ts[kk];
-                        double gal = 0;
+                        // mass: the summed magnitudes of every product inside each
+                        // term (what FP32 rounding of the term is relative to)
+                        double gal = 0, gal_m = 0;
                         for (int ch = 0; ch < 3; ++ch) {
                             a[6 + ch] += alpha * T * gC[ch];
                             am[6 + ch] += std::fabs(alpha * T * gC[ch]);
                             gal += (rgb[g * 4 + ch] - Bk[ch]) * gC[ch];
+                            gal_m += (std::fabs(rgb[g * 4 + ch]) + std::fabs(Bk[ch])) * std::fabs(gC[ch]);
                         }
                         gal *= T;
+                        gal_m *= T;
                         const double A = conic[g * 4], B = conic[g * 4 + 1], C = conic[g * 4 + 2];
-                        const double op = conic[g * 4 + 3];
                         const double dx = xy[g * 2] - x, dy = xy[g * 2 + 1] - y;
                         const double e = std::exp(-0.5 * (A * dx * dx + C * dy * dy) - B * dx * dy);
-                        if (op * e < double(0.99f)) {
-                            const double gpow = alpha * gal;
+                        if (!clampv[kk]) {
+                            const double gpow = alpha * gal, gpow_m = alpha * gal_m;
                             const double t[6] = {gpow * (-(A * dx + B * dy)), gpow * (-(C * dy + B * dx)),
                                                  gpow * (-0.5 * dx * dx), gpow * (-dx * dy), gpow * (-0.5 * dy * dy),
                                                  e * gal};
+                            const double tm[6] = {gpow_m * (std::fabs(A * dx) + std::fabs(B * dy)),
+                                                  gpow_m * (std::fabs(C * dy) + std::fabs(B * dx)),
+                                                  gpow_m * 0.5 * dx * dx, gpow_m * std::fabs(dx * dy),
+                                                  gpow_m * 0.5 * dy * dy, e * gal_m};
                             for (int q = 0; q < 6; ++q) {
                                 a[q] += t[q];
-                                am[q] += std::fabs(t[q]);
+                                am[q] += tm[q];
                             }
                         }
                         for (int ch = 0; ch < 3; ++ch) Bk[ch] = alpha * rgb[g * 4 + ch] + (1 - alpha) * Bk[ch];
@@ -841,7 +864,65 @@ void backward_f64(const float* packed, uint32_t n, const Cam& cam, uint32_t tile
     Packed P{packed, n};
 #pragma omp parallel for schedule(static)
     for (int64_t g = 0; g < (int64_t)n; ++g)
-        preprocess_backward_one(P, (uint32_t)g, cam, &grad2d[g * G2], grad, N, (size_t)g);
+        preprocess_backward_one(P, (uint32_t)g, cam, &grad2d[g * G2], grad, N, (size_t)g,
+                                decisions_f32 ? &flags32[g] : nullptr);
+}
+
+// FP32 replay of the raster adjoint with the same steps as the GPU kernel
+// (T recovered by division, colour-behind recurrence), per-splat sums in
+// double.  A diagnostic twin of the kernel: it separates FP32 term error
+// from summation order.
+void raster_backward_f32(uint32_t W, uint32_t H, uint32_t tile, const uint32_t* ranges, const uint32_t* vals,
+                         const float* conic, const float* xy, const float* rgb, const float* tfinal,
+                         const uint32_t* last, const float* dLdC, uint32_t n, double* grad2d) {
+    const uint32_t TXn = (W + tile - 1) / tile, TYn = (H + tile - 1) / tile;
+    std::fill(grad2d, grad2d + (size_t)n * G2, 0.0);
+    for (uint64_t t = 0; t < (uint64_t)TXn * TYn; ++t) {
+        const uint32_t tx = (uint32_t)(t % TXn), ty = (uint32_t)(t / TXn);
+        const uint32_t start = ranges[t * 2];
+        for (uint32_t ly = 0; ly < tile; ++ly)
+            for (uint32_t lx = 0; lx < tile; ++lx) {
+                const uint32_t x = tx * tile + lx, y = ty * tile + ly;
+                if (x >= W || y >= H) continue;
+                const size_t o = (size_t)y * W + x;
+                float T = tfinal[o];
+                const uint32_t L = last[o];
+                const float gr = dLdC[o], gg = dLdC[(size_t)W * H + o], gb = dLdC[(size_t)2 * W * H + o];
+                float Br = 0, Bg = 0, Bb = 0;
+                for (uint32_t pos = L; pos >= 1; --pos) {
+                    const uint32_t g = vals[start + pos - 1];
+                    const float A = conic[g * 4], B = conic[g * 4 + 1], C = conic[g * 4 + 2], op = conic[g * 4 + 3];
+                    const float dx = xy[g * 2] - (float)x, dy = xy[g * 2 + 1] - (float)y;
+                    const float q = fmaf(A * dx, dx, (C * dy) * dy);
+                    const float power = fmaf(-0.5f, q, -((B * dx) * dy));
+                    if (power > 0.0f || power < -5.55f) continue;
+                    const float e = exp_f32(power);
+                    const float oe = op * e;
+                    const float alpha = std::min(0.99f, oe);
+                    if (alpha < 1.0f / 255.0f) continue;
+                    const float Tb = T / (1.0f - alpha);
+                    T = Tb;
+                    const float cr = rgb[g * 4], cg = rgb[g * 4 + 1], cb = rgb[g * 4 + 2];
+                    const float gal = ((cr - Br) * gr + (cg - Bg) * gg + (cb - Bb) * gb) * Tb;
+                    const float w = alpha * Tb;
+                    double* a = &grad2d[(size_t)g * G2];
+                    a[6] += w * gr; a[7] += w * gg; a[8] += w * gb;
+                    if (oe <= 0.99f) {
+                        const float gpow = alpha * gal;
+                        a[5] += e * gal;
+                        a[2] += gpow * (-0.5f * dx * dx);
+                        a[3] += gpow * (-dx * dy);
+                        a[4] += gpow * (-0.5f * dy * dy);
+                        a[0] += gpow * (-(A * dx + B * dy));
+                        a[1] += gpow * (-(C * dy + B * dx));
+                    }
+                    const float om = 1.0f - alpha;
+                    Br = alpha * cr + om * Br;
+                    Bg = alpha * cg + om * Bg;
+                    Bb = alpha * cb + om * Bb;
+                }
+            }
+    }
 }
 }  // namespace
 
@@ -995,6 +1076,12 @@ void orc_fd_f64(const float* packed, uint32_t n, const void* cam, uint32_t tile,
             same[job] = sig[0] == sig[1] ? 1 : 0;
         }
     }
+}
+
+void orc_raster_backward_f32(uint32_t W, uint32_t H, uint32_t tile, const uint32_t* ranges, const uint32_t* vals,
+                             const float* conic, const float* xy, const float* rgb, const float* tfinal,
+                             const uint32_t* last, const float* dLdC, uint32_t n, double* grad2d) {
+    raster_backward_f32(W, H, tile, ranges, vals, conic, xy, rgb, tfinal, last, dLdC, n, grad2d);
 }
 
 int orc_render_f64(const float* packed, uint32_t n, const void* cam, uint32_t tile, uint64_t cap,

@@ -62,6 +62,8 @@ def lib():
         L.orc_param_mass_f64.argtypes = [_f32p, ctypes.c_uint32, _f32p, _f64p, _f64p]
         L.orc_fd_f64.argtypes = [_f32p, ctypes.c_uint32, _f32p, ctypes.c_uint32, _f64p, _u32p, ctypes.c_uint32,
                                  ctypes.c_float, _f64p, np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")]
+        L.orc_raster_backward_f32.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p, _f32p,
+                                              _f32p, _f32p, _f32p, _u32p, _f32p, ctypes.c_uint32, _f64p]
         L.orc_preprocess_backward_f64.argtypes = [_f32p, ctypes.c_uint32, _f32p, _f64p, _f64p]
         _lib = L
     return _lib
@@ -199,3 +201,14 @@ def finite_differences(packed, n, uni, dLdC, splats, eps=1e-4, tile=16):
     lib().orc_fd_f64(np.ascontiguousarray(packed, np.float32), n, np.ascontiguousarray(uni, np.float32), tile,
                      np.ascontiguousarray(dLdC, np.float64).ravel(), splats, k, eps, fd, same)
     return fd.reshape(k, 59).T, same.reshape(k, 59).T.astype(bool)
+
+
+def raster_backward_f32(W, H, tile, ranges, vals, conic, xy, rgb, tfinal, last, dLdC, n):
+    """FP32 replay of the GPU raster adjoint (diagnostic twin); returns grad2d [n, 12] (double sums)."""
+    g2 = np.zeros(12 * n, np.float64)
+    lib().orc_raster_backward_f32(W, H, tile, np.ascontiguousarray(ranges, np.uint32), np.ascontiguousarray(vals, np.uint32),
+                                  np.ascontiguousarray(conic, np.float32).ravel(), np.ascontiguousarray(xy, np.float32).ravel(),
+                                  np.ascontiguousarray(rgb, np.float32).ravel(), np.ascontiguousarray(tfinal, np.float32).ravel(),
+                                  np.ascontiguousarray(last, np.uint32).ravel(), np.ascontiguousarray(dLdC, np.float32).ravel(),
+                                  n, g2)
+    return g2.reshape(n, 12)

@@ -2,15 +2,19 @@
 
 The FP64 adjoint (oracle, decisions taken from the FP32 forward) is itself
 pinned by FP64 central finite differences in tests/test_oracle_adjoint.py.
-Tolerance: the GPU gradients must agree with it within 1e-3 relative,
-norm-wise per parameter group, and element-wise as
-    |g - g64| <= 1e-3 |g64| + 1e-4 m
-where m is the summed magnitude of the terms the gradient adds up (the FP64
-adjoint reports it; for parameters it is |J|^T of the 2-D masses).  The m term
-covers gradients whose per-pixel terms cancel, where FP32 summation noise
-makes a pure relative bound meaningless.  99.9% of elements must meet the
-bound and all must meet it ten times over.  The forward with saved state
-must also be bit-identical to the plain forward."""
+Tolerances.
+* Against the FP64 adjoint: within 1e-3 relative norm-wise per parameter
+  group, and element-wise |g - g64| <= 1e-3 |g64| + 1e-4 m for 99.9% of the
+  elements, where m is the summed magnitude of every product the gradient's
+  terms are built from (the FP64 adjoint reports it; for parameters it is
+  |J|^T of the 2-D masses).  The last 0.1% are terms FP32 cannot resolve,
+  e.g. a splat centred within 1e-3 px of a pixel centre, where x_splat -
+  x_pixel loses most of its bits.
+* Against the FP32 twin of the raster adjoint (oracle.raster_backward_f32,
+  the kernel's arithmetic replayed on the CPU with double sums): every
+  element within 1e-4 m.  Only the summation order (wave reductions, atomics)
+  differs, so this pins the kernel itself.
+The forward with saved state must be bit-identical to the plain forward."""
 import sys
 
 import numpy as np
@@ -62,7 +66,24 @@ def _compare(name, got, ref, mass, axis_groups, axis):
              f"max {ratio.max():.3f}")
         assert rel < 1e-3, (gname, rel)
         assert np.quantile(ratio, 0.999) <= 1.0, (gname, np.quantile(ratio, 0.999))
-        assert ratio.max() <= 10.0, (gname, ratio.max())
+
+
+def _compare_twin(torch, ctx, orc, sc, u, dLdC, g2, m2, tile=16):
+    """GPU 2-D gradients vs the FP32 CPU twin on the same (bit-exact) forward."""
+    n = sc.count
+    W, H = int(u[40]), int(u[41])
+    pre = orc.preprocess(sc.packed(), n, u, tile=tile)
+    offs, total = orc.scan(np.concatenate([pre["tiles"], [0]]).astype(np.uint32))
+    keys, vals = orc.emit(offs, pre["meta"], n, W, tile, 64 * n)
+    ks, vs = orc.sort_radix(keys, vals, 40)
+    TX, TY = (W + tile - 1) // tile, (H + tile - 1) // tile
+    ranges = orc.tile_ranges(ks, TX * TY)
+    _, T, last = orc.raster(W, H, tile, ranges, vs, pre["conic"], pre["xy"], pre["rgb"])
+    twin = orc.raster_backward_f32(W, H, tile, ranges, vs, pre["conic"], pre["xy"], pre["rgb"], T, last,
+                                   np.asarray(dLdC, np.float32), n)
+    err = np.abs(g2[:, :9] - twin[:, :9]) / np.maximum(1e-4 * m2[:, :9], 1e-30)
+    _say(f"twin: error / (1e-4 mass) max {err.max():.3f}")
+    assert err.max() <= 1.0, err.max()
 
 
 @pytest.mark.parametrize("workload", ["small", "clustered"])
@@ -80,6 +101,7 @@ def test_backward_matches_fp64_adjoint(torch_cuda, ctx, orc, workload):
     ref, ref2, _, m2, pm = orc.backward(sc.packed(), sc.count, u, dLdC, decisions_f32=True, want_mass=True)
     _compare("grad2d", g2, ref2, m2, G2, axis=1)
     _compare("params", grad, ref, pm, GROUPS, axis=0)
+    _compare_twin(torch_cuda, ctx, orc, sc, u, dLdC, g2, m2)
 
 
 def test_forward_with_state_is_bit_identical(torch_cuda, ctx):
@@ -132,3 +154,4 @@ def test_backward_1m_1080p(torch_cuda, ctx, orc):
     ref, ref2, _, m2, pm = orc.backward(sc.packed(), sc.count, u, dLdC, decisions_f32=True, want_mass=True)
     _compare("grad2d", g2, ref2, m2, G2, axis=1)
     _compare("params", grad, ref, pm, GROUPS, axis=0)
+    _compare_twin(torch_cuda, ctx, orc, sc, u, dLdC, g2, m2)

@@ -451,10 +451,4 @@ int gsr_backward(gsr_context* ctx, const float* params_dev, uint32_t count, cons
     return launch_backward(c, params_dev, count, ck, dL_drgb_dev, grad_params_dev, grad2d_dev);
 }
 
-int gsr_render_batch(gsr_context* ctx, const float*, uint32_t, const gsr_camera*, uint32_t, const gsr_options*, int,
-                     float*, uint64_t*) {
-    Context* c = reinterpret_cast<Context*>(ctx);
-    return fail(c, GSR_ERR_STATE, "gsr_render_batch: not available in this build");
-}
-
 }  // extern "C"

@@ -29,6 +29,7 @@ void* buffer(Context* c, const char* name, size_t bytes) {
     DevBuf& b = c->bufs[name];
     if (b.bytes >= bytes) return b.ptr;
     if (b.ptr) {
+        destroy_batch_graph(c);
         hipStreamSynchronize(c->stream);
         hipFree(b.ptr);
         b.ptr = nullptr;
@@ -42,6 +43,10 @@ void* buffer(Context* c, const char* name, size_t bytes) {
     }
     b.bytes = rounded;
     return b.ptr;
+}
+
+void* scratch(Context* c, const char* name, size_t bytes) {
+    return buffer(c, (std::string(c->scratch_prefix) + name).c_str(), bytes);
 }
 
 static hipEvent_t next_event(Context* c) {
@@ -177,6 +182,8 @@ void gsr_destroy(gsr_context* ctx) {
     if (!c) return;
     hipSetDevice(c->device);
     hipStreamSynchronize(c->stream);
+    destroy_batch_graph(c);
+    if (c->staging) hipHostFree(c->staging);
     for (auto& kv : c->scenes) hipFree(kv.second.params);
     for (auto& kv : c->bufs) hipFree(kv.second.ptr);
     for (hipEvent_t e : c->event_pool) hipEventDestroy(e);

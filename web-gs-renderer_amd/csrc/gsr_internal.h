@@ -102,6 +102,16 @@ struct TimingAcc {
     int frames = 0;
 };
 
+// A captured batch frame and the shape it was captured for.
+struct BatchGraph {
+    hipGraphExec_t exec = nullptr;
+    uint32_t count = 0, views = 0, tile = 0, key_bytes = 0, raster_waves = 0, sort_blocks = 0;
+    uint64_t capacity = 0;
+    const float* params = nullptr;
+    const float* out = nullptr;
+    std::vector<uint64_t> dims;  // (W << 32) | H per view
+};
+
 struct Context {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -124,12 +134,25 @@ struct Context {
     uint64_t state_capacity = 0;
     int state_sorted_index = 0;
     uint32_t last_frame_w = 0, last_frame_h = 0;  // size of the frame in "image"
+    // launch shape knobs (set per call from gsr_options)
+    uint32_t sort_blocks_per_cu = 2;
+    // prefix of the scan/sort scratch buffer names ("" plain frames, "b_" batches)
+    const char* scratch_prefix = "";
+    // batches
+    void* staging = nullptr;   // pinned per-view tables
+    size_t staging_bytes = 0;
+    uint64_t batch_capacity_hint = 0, batch_last_instances = 0;
+    BatchGraph graph;
     // host pinned staging for small readbacks
     FrameCounters* host_counters = nullptr;
 };
 
 // Named device buffer, grown to at least `bytes` (contents not preserved).
+// Growing a buffer drops any captured batch graph (it may point at it).
 void* buffer(Context* c, const char* name, size_t bytes);
+// Scratch buffer named scratch_prefix + name.
+void* scratch(Context* c, const char* name, size_t bytes);
+void destroy_batch_graph(Context* c);
 
 int fail(Context* c, int code, const char* fmt, ...);
 #define GSR_HIP(ctx, call)                                                                 \
@@ -152,6 +175,10 @@ void finish_frame_timings(Context* c);
 // ---- kernel launchers (each enqueues on c->stream) -------------------------
 int launch_preprocess(Context* c, const float* params, uint32_t n, const CamK& cam, uint32_t tile,
                       uint32_t* tiles, uint4* meta, float4* conic, float2* xy, float4* rgb);
+int launch_preprocess_batch(Context* c, const float* params, uint32_t n, const CamK* cams, uint32_t views,
+                            uint32_t tile, uint32_t* tiles, uint4* meta, float4* conic, float2* xy, float4* rgb);
+int launch_emit_batch(Context* c, const uint32_t* offsets, const uint4* meta, uint32_t n, uint32_t num_views,
+                      const uint4* views, uint64_t capacity, uint32_t key_bytes, void* keys, uint32_t* vals);
 int launch_scan(Context* c, const uint32_t* in, uint32_t n, uint32_t* out, FrameCounters* counters,
                 uint64_t capacity);
 int launch_emit(Context* c, const uint32_t* offsets, const uint4* meta, uint32_t n, uint32_t tiles_x,
@@ -167,6 +194,11 @@ int launch_raster(Context* c, uint32_t W, uint32_t H, uint32_t tile, uint32_t sp
                   uint32_t n_host, const uint32_t* ranges, const uint32_t* vals, const float4* conic,
                   const float2* xy, const float4* rgb, float* out, float* tfinal, uint32_t* last);
 int launch_fill_u32(Context* c, uint32_t* p, uint32_t value, size_t count);
+// Batch raster: views[v] = {W, H, tiles_x, tile_base}, pix[v] = float offset of view v's image.
+int launch_raster_queue(Context* c, uint32_t tile, const uint4* views, const uint64_t* pix, uint32_t num_views,
+                        uint32_t total_tiles, uint32_t splats, const uint32_t* n_dev, uint32_t* ticket,
+                        const uint32_t* ranges, const uint32_t* vals, const float4* conic, const float2* xy,
+                        const float4* rgb, float* out, uint32_t waves_per_cu);
 
 __host__ __device__ inline uint32_t ceil_div(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 

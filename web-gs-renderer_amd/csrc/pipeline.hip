@@ -12,7 +12,7 @@
 namespace gsr {
 
 struct FramePlan {
-    uint32_t W, H, tile, tiles_x, tiles_y, num_tiles, key_bytes, end_bit;
+    uint32_t W, H, tile, tiles_x, tiles_y, num_tiles, key_bytes, end_bit, sort_blocks;
     uint64_t capacity;
 };
 
@@ -47,6 +47,7 @@ int plan_frame(Context* c, uint32_t count, const gsr_camera* cam, const gsr_opti
         cap = kMaxSortKeys;
     }
     p->capacity = cap;
+    p->sort_blocks = (opts && opts->sort_blocks_per_cu) ? opts->sort_blocks_per_cu : 2;
     return GSR_OK;
 }
 
@@ -55,6 +56,7 @@ int plan_frame(Context* c, uint32_t count, const gsr_camera* cam, const gsr_opti
 int enqueue_forward(Context* c, const float* params, uint32_t count, const gsr_camera* cam, const FramePlan& p,
                     bool save_state, float* image_override) {
     const size_t N = count;
+    c->sort_blocks_per_cu = p.sort_blocks;
     uint32_t* tiles = (uint32_t*)buffer(c, "tiles", N * 4);
     uint32_t* offsets = (uint32_t*)buffer(c, "offsets", (N + 1) * 4);
     uint4* meta = (uint4*)buffer(c, "meta", N * 16);

@@ -21,26 +21,21 @@ constexpr float SH_C3_0 = -0.5900435899266435f, SH_C3_1 = 2.890611442640554f,
                 SH_C3_4 = -0.4570457994644658f, SH_C3_5 = 1.445305721320277f,
                 SH_C3_6 = -0.5900435899266435f;
 
-__global__ __launch_bounds__(256) void k_preprocess(const float* __restrict__ P, uint32_t n, CamK cam,
-                                                    uint32_t tile, uint32_t* __restrict__ tiles_out,
-                                                    uint4* __restrict__ meta_out,
-                                                    float4* __restrict__ conic_out,
-                                                    float2* __restrict__ xy_out,
-                                                    float4* __restrict__ rgb_out) {
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= n) return;
-    const size_t N = n;
+// One splat seen by one camera.  Outputs as gsr_preprocess documents them.
+__device__ __forceinline__ void preprocess_splat(const float* __restrict__ P, size_t N, uint32_t g, const CamK& cam,
+                                                 uint32_t tile, uint32_t& ntiles, uint4& meta, float4& conic,
+                                                 float2& xy, float4& rgb) {
     const float px = P[CH_POS * N + g], py = P[(CH_POS + 1) * N + g], pz = P[(CH_POS + 2) * N + g];
     const float* V = cam.view;
     const float* PV = cam.view_proj;
     const float tx = ((V[0] * px + V[4] * py) + V[8] * pz) + V[12];
     const float ty = ((V[1] * px + V[5] * py) + V[9] * pz) + V[13];
     const float tz = ((V[2] * px + V[6] * py) + V[10] * pz) + V[14];
-    uint32_t ntiles = 0;
-    uint4 meta = make_uint4(0, 0, 0, 0);
-    float4 conic = make_float4(0.f, 0.f, 0.f, 0.f);
-    float2 xy = make_float2(0.f, 0.f);
-    float4 rgb = make_float4(0.f, 0.f, 0.f, 0.f);
+    ntiles = 0;
+    meta = make_uint4(0, 0, 0, 0);
+    conic = make_float4(0.f, 0.f, 0.f, 0.f);
+    xy = make_float2(0.f, 0.f);
+    rgb = make_float4(0.f, 0.f, 0.f, 0.f);
     if (tz < -kNearView) {
         const float cx = ((PV[0] * px + PV[4] * py) + PV[8] * pz) + PV[12];
         const float cy = ((PV[1] * px + PV[5] * py) + PV[9] * pz) + PV[13];
@@ -153,6 +148,22 @@ __global__ __launch_bounds__(256) void k_preprocess(const float* __restrict__ P,
             meta = make_uint4((y0 << 16) | x0, (y1 << 16) | x1, depth16, 0u);
         }
     }
+}
+
+__global__ __launch_bounds__(256) void k_preprocess(const float* __restrict__ P, uint32_t n, CamK cam,
+                                                    uint32_t tile, uint32_t* __restrict__ tiles_out,
+                                                    uint4* __restrict__ meta_out,
+                                                    float4* __restrict__ conic_out,
+                                                    float2* __restrict__ xy_out,
+                                                    float4* __restrict__ rgb_out) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    uint32_t ntiles;
+    uint4 meta;
+    float4 conic;
+    float2 xy;
+    float4 rgb;
+    preprocess_splat(P, n, g, cam, tile, ntiles, meta, conic, xy, rgb);
     tiles_out[g] = ntiles;
     meta_out[g] = meta;
     conic_out[g] = conic;
@@ -160,7 +171,45 @@ __global__ __launch_bounds__(256) void k_preprocess(const float* __restrict__ P,
     rgb_out[g] = rgb;
 }
 
+// View batches: one thread per splat walks every camera, so a splat's
+// parameters are fetched from HBM once for all views; outputs of view v sit
+// at v * n + g.
+__global__ __launch_bounds__(256) void k_preprocess_batch(const float* __restrict__ P, uint32_t n,
+                                                          const CamK* __restrict__ cams, uint32_t views,
+                                                          uint32_t tile, uint32_t* __restrict__ tiles_out,
+                                                          uint4* __restrict__ meta_out,
+                                                          float4* __restrict__ conic_out,
+                                                          float2* __restrict__ xy_out,
+                                                          float4* __restrict__ rgb_out) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    for (uint32_t v = 0; v < views; ++v) {
+        const CamK cam = cams[v];
+        uint32_t ntiles;
+        uint4 meta;
+        float4 conic;
+        float2 xy;
+        float4 rgb;
+        preprocess_splat(P, n, g, cam, tile, ntiles, meta, conic, xy, rgb);
+        const size_t o = (size_t)v * n + g;
+        tiles_out[o] = ntiles;
+        meta_out[o] = meta;
+        conic_out[o] = conic;
+        xy_out[o] = xy;
+        rgb_out[o] = rgb;
+    }
+}
+
 }  // namespace
+
+int launch_preprocess_batch(Context* c, const float* params, uint32_t n, const CamK* cams, uint32_t views,
+                            uint32_t tile, uint32_t* tiles, uint4* meta, float4* conic, float2* xy, float4* rgb) {
+    if (n == 0 || views == 0) return GSR_OK;
+    hipLaunchKernelGGL(k_preprocess_batch, dim3(ceil_div(n, 256)), dim3(256), 0, c->stream, params, n, cams, views,
+                       tile, tiles, meta, conic, xy, rgb);
+    GSR_HIP(c, hipGetLastError());
+    return GSR_OK;
+}
 
 int launch_preprocess(Context* c, const float* params, uint32_t n, const CamK& cam, uint32_t tile,
                       uint32_t* tiles, uint4* meta, float4* conic, float2* xy, float4* rgb) {

@@ -51,38 +51,32 @@ __device__ __forceinline__ float bcast(float v, int lane) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
+// One tile of one view, run by one wave64.  (x0, y0) is the tile's first
+// pixel; out/tfinal/last address the view's image ([3][H][W] planes).
 template <int TS>
-__global__ __launch_bounds__(64) void k_raster(uint32_t W, uint32_t H, uint32_t tiles_x, uint32_t splats,
-                                               const uint32_t* n_dev, uint32_t n_host,
-                                               const uint2* __restrict__ ranges,
-                                               const uint32_t* __restrict__ vals,
-                                               const float4* __restrict__ conic,
-                                               const float2* __restrict__ xy,
-                                               const float4* __restrict__ rgb, float* __restrict__ out,
-                                               float* __restrict__ tfinal, uint32_t* __restrict__ last) {
+__device__ __forceinline__ void raster_tile(uint32_t W, uint32_t H, uint32_t x0, uint32_t y0, uint32_t start,
+                                            uint32_t end, uint32_t splats, const uint32_t* __restrict__ vals,
+                                            const float4* __restrict__ conic, const float2* __restrict__ xy,
+                                            const float4* __restrict__ rgb, float* __restrict__ out,
+                                            float* __restrict__ tfinal, uint32_t* __restrict__ last) {
     constexpr int PPL = TS * TS / 64;
     constexpr int STEP = 64 / TS;
-    const uint32_t tile = blockIdx.x;
-    const uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
-    const uint32_t lane = threadIdx.x;
-    const uint32_t x = tx * TS + lane % TS;
-    const uint32_t y0 = ty * TS + lane / TS;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t x = x0 + lane % TS;
+    const uint32_t yb = y0 + lane / TS;
     const float fx = (float)x;
     float fy[PPL], T[PPL], c0[PPL], c1[PPL], c2[PPL];
     uint32_t lst[PPL];
     bool done[PPL];
 #pragma unroll
     for (int k = 0; k < PPL; ++k) {
-        const uint32_t y = y0 + k * STEP;
+        const uint32_t y = yb + k * STEP;
         done[k] = !(x < W && y < H);
         fy[k] = (float)y;
         T[k] = 1.0f;
         c0[k] = c1[k] = c2[k] = 0.0f;
         lst[k] = 0;
     }
-    const uint2 r = ranges[tile];
-    const uint32_t n_inst = n_dev ? *n_dev : n_host;
-    const uint32_t start = r.x, end = min(r.y, n_inst);  // ranges never exceed n; bounds the reads
     for (uint32_t b = start; b < end; b += 64) {
         bool active = false;
 #pragma unroll
@@ -133,7 +127,7 @@ __global__ __launch_bounds__(64) void k_raster(uint32_t W, uint32_t H, uint32_t 
     const size_t plane = (size_t)W * H;
 #pragma unroll
     for (int k = 0; k < PPL; ++k) {
-        const uint32_t y = y0 + k * STEP;
+        const uint32_t y = yb + k * STEP;
         if (x < W && y < H) {
             const size_t o = (size_t)y * W + x;
             out[o] = c0[k];
@@ -142,6 +136,57 @@ __global__ __launch_bounds__(64) void k_raster(uint32_t W, uint32_t H, uint32_t 
             if (tfinal) tfinal[o] = T[k];
             if (last) last[o] = lst[k];
         }
+    }
+}
+
+template <int TS>
+__global__ __launch_bounds__(64) void k_raster(uint32_t W, uint32_t H, uint32_t tiles_x, uint32_t splats,
+                                               const uint32_t* n_dev, uint32_t n_host,
+                                               const uint2* __restrict__ ranges,
+                                               const uint32_t* __restrict__ vals,
+                                               const float4* __restrict__ conic,
+                                               const float2* __restrict__ xy,
+                                               const float4* __restrict__ rgb, float* __restrict__ out,
+                                               float* __restrict__ tfinal, uint32_t* __restrict__ last) {
+    const uint32_t tile = blockIdx.x;
+    const uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
+    const uint2 r = ranges[tile];
+    const uint32_t n_inst = n_dev ? *n_dev : n_host;
+    const uint32_t end = min(r.y, n_inst);  // ranges never exceed n; bounds the reads
+    raster_tile<TS>(W, H, tx * TS, ty * TS, r.x, end, splats, vals, conic, xy, rgb, out, tfinal, last);
+}
+
+// Persistent form for view batches: each wave pulls global tile ids from a
+// ticket counter until all are done.  views[v] = {W, H, tiles_x, tile_base}
+// and pix[v] the float offset of view v's image.
+template <int TS>
+__global__ __launch_bounds__(64) void k_raster_queue(const uint4* __restrict__ views,
+                                                     const uint64_t* __restrict__ pix, uint32_t num_views,
+                                                     uint32_t total_tiles, uint32_t splats, const uint32_t* n_dev,
+                                                     uint32_t* ticket, const uint2* __restrict__ ranges,
+                                                     const uint32_t* __restrict__ vals,
+                                                     const float4* __restrict__ conic,
+                                                     const float2* __restrict__ xy, const float4* __restrict__ rgb,
+                                                     float* __restrict__ out) {
+    const uint32_t n_inst = *n_dev;
+    for (;;) {
+        uint32_t t = 0;
+        if ((threadIdx.x & 63) == 0) t = atomicAdd(ticket, 1u);
+        t = (uint32_t)__builtin_amdgcn_readfirstlane((int)t);
+        if (t >= total_tiles) break;
+        // view holding global tile t: last v with tile_base <= t
+        uint32_t lo = 0, hi = num_views;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (views[mid].w <= t) lo = mid;
+            else hi = mid;
+        }
+        const uint4 vi = views[lo];
+        const uint32_t local = t - vi.w;
+        const uint32_t tx = local % vi.z, ty = local / vi.z;
+        const uint2 r = ranges[t];
+        raster_tile<TS>(vi.x, vi.y, tx * TS, ty * TS, r.x, min(r.y, n_inst), splats, vals, conic, xy, rgb,
+                        out + pix[lo], nullptr, nullptr);
     }
 }
 
@@ -168,6 +213,33 @@ int launch_ranges(Context* c, const void* keys, uint32_t key_bytes, const uint32
     else
         hipLaunchKernelGGL(k_ranges<uint32_t>, dim3(blocks), dim3(256), 0, c->stream, (const uint32_t*)keys,
                            n_dev, n_host, num_tiles, ranges);
+    GSR_HIP(c, hipGetLastError());
+    return GSR_OK;
+}
+
+int launch_raster_queue(Context* c, uint32_t tile, const uint4* views, const uint64_t* pix, uint32_t num_views,
+                        uint32_t total_tiles, uint32_t splats, const uint32_t* n_dev, uint32_t* ticket,
+                        const uint32_t* ranges, const uint32_t* vals, const float4* conic, const float2* xy,
+                        const float4* rgb, float* out, uint32_t waves_per_cu) {
+    if (total_tiles == 0) return GSR_OK;
+    const uint32_t waves = std::min<uint32_t>(total_tiles, (uint32_t)c->cu_count * (waves_per_cu ? waves_per_cu : 16));
+    const uint2* rg = (const uint2*)ranges;
+    switch (tile) {
+        case 8:
+            hipLaunchKernelGGL(k_raster_queue<8>, dim3(waves), dim3(64), 0, c->stream, views, pix, num_views,
+                               total_tiles, splats, n_dev, ticket, rg, vals, conic, xy, rgb, out);
+            break;
+        case 16:
+            hipLaunchKernelGGL(k_raster_queue<16>, dim3(waves), dim3(64), 0, c->stream, views, pix, num_views,
+                               total_tiles, splats, n_dev, ticket, rg, vals, conic, xy, rgb, out);
+            break;
+        case 32:
+            hipLaunchKernelGGL(k_raster_queue<32>, dim3(waves), dim3(64), 0, c->stream, views, pix, num_views,
+                               total_tiles, splats, n_dev, ticket, rg, vals, conic, xy, rgb, out);
+            break;
+        default:
+            return fail(c, GSR_ERR_ARGUMENT, "tile size %u not in {8, 16, 32}", tile);
+    }
     GSR_HIP(c, hipGetLastError());
     return GSR_OK;
 }

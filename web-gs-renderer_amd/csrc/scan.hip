@@ -96,7 +96,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(const uint32_t* in, uint3
 int launch_scan(Context* c, const uint32_t* in, uint32_t n, uint32_t* out, FrameCounters* counters,
                 uint64_t capacity) {
     const uint32_t tiles = ceil_div((uint64_t)n + 1, kScanTile);
-    uint64_t* status = (uint64_t*)buffer(c, "scan_status", (size_t)(tiles + 1) * 8);
+    uint64_t* status = (uint64_t*)scratch(c, "scan_status", (size_t)(tiles + 1) * 8);
     if (!status) return GSR_ERR_ALLOC;
     int rc = launch_fill_u32(c, (uint32_t*)status, 0u, (size_t)(tiles + 1) * 2);
     if (rc) return rc;

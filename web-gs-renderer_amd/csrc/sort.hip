@@ -211,8 +211,8 @@ int sort_impl(Context* c, K* k0, uint32_t* v0, K* k1, uint32_t* v1, const uint32
     if (passes == 0) return GSR_OK;
     if (passes > (uint32_t)kMaxPasses) return fail(c, GSR_ERR_ARGUMENT, "too many radix passes (%u)", passes);
     const uint32_t max_tiles = std::max<uint32_t>(1, ceil_div(capacity, kSortTile));
-    uint32_t* hist = (uint32_t*)buffer(c, "sort_hist", ((size_t)kMaxPasses * kRadix + kMaxPasses) * 4);
-    uint32_t* status = (uint32_t*)buffer(c, "sort_status", (size_t)passes * max_tiles * kRadix * 4);
+    uint32_t* hist = (uint32_t*)scratch(c, "sort_hist", ((size_t)kMaxPasses * kRadix + kMaxPasses) * 4);
+    uint32_t* status = (uint32_t*)scratch(c, "sort_status", (size_t)passes * max_tiles * kRadix * 4);
     if (!hist || !status) return GSR_ERR_ALLOC;
     uint32_t* tickets = hist + kMaxPasses * kRadix;
     const uint32_t fill_blocks = std::min<uint32_t>(std::max<uint32_t>(1, max_tiles), 1024);
@@ -222,7 +222,8 @@ int sort_impl(Context* c, K* k0, uint32_t* v0, K* k1, uint32_t* v1, const uint32
                                                     (uint32_t)c->cu_count * 4);
     hipLaunchKernelGGL(k_sort_hist<K>, dim3(hist_blocks), dim3(256), 0, c->stream, k0, n_dev, n_host,
                        begin_bit, end_bit, passes, hist);
-    const uint32_t sweep_blocks = std::min<uint32_t>(max_tiles, (uint32_t)c->cu_count * 2);
+    const uint32_t sweep_blocks =
+        std::min<uint32_t>(max_tiles, (uint32_t)c->cu_count * std::max<uint32_t>(1, c->sort_blocks_per_cu));
     K* kb[2] = {k0, k1};
     uint32_t* vb[2] = {v0, v1};
     for (uint32_t p = 0; p < passes; ++p) {

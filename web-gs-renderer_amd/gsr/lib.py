@@ -48,7 +48,8 @@ assert ctypes.sizeof(Camera) == 176
 class Options(ctypes.Structure):
     _fields_ = [("tile_size", ctypes.c_uint32), ("key_bytes", ctypes.c_uint32),
                 ("instance_capacity", ctypes.c_uint64), ("timing", ctypes.c_uint32),
-                ("save_state", ctypes.c_uint32)]
+                ("save_state", ctypes.c_uint32), ("raster_waves_per_cu", ctypes.c_uint32),
+                ("sort_blocks_per_cu", ctypes.c_uint32)]
 
 
 class TimingRow(ctypes.Structure):
@@ -104,6 +105,9 @@ def load():
     L.gsr_backward.argtypes = [_VP, _VP, _U32, ctypes.POINTER(Camera), _VP, _VP, _VP]
     L.gsr_render_batch.argtypes = [_VP, _VP, _U32, ctypes.POINTER(Camera), _U32, ctypes.POINTER(Options), _INT,
                                    _VP, ctypes.POINTER(_U64)]
+    L.gsr_batch_instance_count.argtypes = [_VP, ctypes.POINTER(_U64)]
+    L.gsr_autotune_batch.argtypes = [_VP, _VP, _U32, ctypes.POINTER(Camera), _U32, ctypes.POINTER(Options), _VP,
+                                     ctypes.POINTER(ctypes.c_double)]
     special = {"gsr_destroy": None, "gsr_last_error": ctypes.c_char_p, "gsr_stream": _VP}
     for name in header_functions():
         getattr(L, name).restype = special.get(name, ctypes.c_int)
@@ -115,8 +119,10 @@ def last_error(ctx=None) -> str:
     return load().gsr_last_error(ctx).decode(errors="replace")
 
 
-def options(tile_size=16, key_bytes=KEY_AUTO, capacity=0, timing=False, save_state=False) -> Options:
-    return Options(tile_size, key_bytes, capacity, 1 if timing else 0, 1 if save_state else 0)
+def options(tile_size=16, key_bytes=KEY_AUTO, capacity=0, timing=False, save_state=False, raster_waves_per_cu=0,
+            sort_blocks_per_cu=0) -> Options:
+    return Options(tile_size, key_bytes, capacity, 1 if timing else 0, 1 if save_state else 0, raster_waves_per_cu,
+                   sort_blocks_per_cu)
 
 
 def _ptr(x):
@@ -212,6 +218,26 @@ class Context:
                                             ctypes.byref(opts) if opts else None, 1 if use_graph else 0,
                                             _ptr(out_dev), offs))
         return list(offs)
+
+    def batch_layout(self, count: int, uniforms: list, opts: Options | None = None):
+        """Float offsets of each view's image in a batch output (views + 1 entries)."""
+        cams = (Camera * len(uniforms))(*[Camera.from_uniform(u) for u in uniforms])
+        offs = (_U64 * (len(uniforms) + 1))()
+        self._check(self.L.gsr_render_batch(self.h, 1, count, cams, len(uniforms),
+                                            ctypes.byref(opts) if opts else None, 0, None, offs))
+        return list(offs)
+
+    def batch_instance_count(self) -> int:
+        v = _U64()
+        self._check(self.L.gsr_batch_instance_count(self.h, ctypes.byref(v)))
+        return v.value
+
+    def autotune_batch(self, params_dev, count: int, uniforms: list, opts: Options, out_dev):
+        cams = (Camera * len(uniforms))(*[Camera.from_uniform(u) for u in uniforms])
+        best = ctypes.c_double()
+        self._check(self.L.gsr_autotune_batch(self.h, _ptr(params_dev), count, cams, len(uniforms), ctypes.byref(opts),
+                                              _ptr(out_dev), ctypes.byref(best)))
+        return opts, best.value
 
     def instance_count(self) -> int:
         v = _U64()
