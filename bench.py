@@ -23,6 +23,10 @@ sys.path.insert(0, os.path.join(ROOT, "web-gs-renderer_amd"))
 
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
+# HBM bytes per launch of the forward's kernels at this workload, from the
+# rocprofv3 FETCH_SIZE / WRITE_SIZE runs summarised in
+# profiles/forward_1m_1080p_pmc_traffic.md (calibrated against a plain copy).
+MEASURED_TRAFFIC = {"raster": 850439425, "preprocess": 296095744, "sort": 4 * 60755968 + 15953100}
 
 
 def log(msg):
@@ -58,15 +62,16 @@ def cpu_baseline(sc, u, budget_s: float = 15.0):
     frames = 0
     t0 = time.perf_counter()
     packed = sc.packed()
+    evals = []
     while True:
-        oracle.render(packed, sc.count, u, use_radix=True)
+        oracle.render(packed, sc.count, u, use_radix=True, evals=evals)
         frames += 1
         if time.perf_counter() - t0 > budget_s or frames >= 8:
             break
     dt = time.perf_counter() - t0
     return {"value": frames / dt, "unit": "frames/s", "cores": threads, "kind": "port",
             "sample": f"{frames} full frames of the same 1M-splat 1080p workload through the oracle's FP32 "
-                      f"C++ pass graph (OpenMP, {threads} threads, stable LSD radix sort)"}
+                      f"C++ pass graph (OpenMP, {threads} threads, stable LSD radix sort)"}, evals[-1]
 
 
 def batch_leg(ctx, torch, lib, scenes, reps=10):
@@ -119,6 +124,10 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--forward-only", action="store_true",
+                    help="time only the plain forward (for short profiler runs); the JSON line has no extra legs")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB five times (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the renderer has no "
@@ -132,6 +141,14 @@ def main():
     if not torch.cuda.is_available():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.copy_probe:
+        src = torch.ones(args.copy_probe * (1 << 20) // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": src.numel() * 4, "copies": 5}))
+        return
     log("building the 1M-splat scene")
     sc, cam = scenes.workload_1m_1080p()
     u = cam.uniform(sc.count)
@@ -187,8 +204,8 @@ def main():
     dom_bytes = algorithmic_bytes(dominant, n, instances, visible, W, H, tiles, key_bytes, passes)
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": dominant,
-                "kernel_ms": round(dom_ms, 5), "algorithmic_bytes": int(dom_bytes)}
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": MEASURED_TRAFFIC.get(dominant),
+                "kernel": dominant, "kernel_ms": round(dom_ms, 5), "algorithmic_bytes": int(dom_bytes)}
     sort_ms = stages.get("sort", 0.0)
     extra = {
         "stage_ms": {k: round(v, 5) for k, v in stage_ms.items()},
@@ -198,6 +215,11 @@ def main():
         "sort_gbs": round(algorithmic_bytes("sort", n, instances, visible, W, H, tiles, key_bytes, passes)
                           / (sort_ms * 1e-3) / 1e9, 2) if sort_ms else None,
     }
+    if args.forward_only:
+        ctx.close()
+        print(json.dumps({"metric": "forward frames/s (profiling run)", "value": round(fps, 3),
+                          "ms_per_step": round(ms_per_step, 5), "stage_ms": extra["stage_ms"]}))
+        return
     # forward + backward (config: same scene, upstream dL/dRGB ~ N(0,1) resident in HBM)
     g = torch.Generator(device="cuda").manual_seed(2)
     dldc = torch.randn(3 * H * W, device="cuda", generator=g)
@@ -226,7 +248,13 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("cpu baseline")
-        cpu = cpu_baseline(sc, u)
+        cpu, evals = cpu_baseline(sc, u)
+        # the raster is VALU-bound: its algorithmic work is the pixel-splat
+        # evaluations under the early-termination rules (counted by the oracle)
+        rms = stages.get("raster", 0.0)
+        if rms:
+            extra["raster_valu"] = {"evaluations": evals, "evaluations_per_s": round(evals / (rms * 1e-3), 1),
+                                    "note": "pixel-splat evaluations per raster launch / raster time"}
     ctx.close()
     out = {
         "metric": "fwd & fwd+bwd frames/s @1M splats 1080p; 64-view batch vs plain; sort Gkeys/s",

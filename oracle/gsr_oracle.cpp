@@ -398,13 +398,16 @@ inline uint64_t fnv(uint64_t h, uint64_t v) {
 // sig (optional, per pixel): hash of the pixel's discrete decisions (which
 // splats it blends and where it stops), to tell when two frames lie on the
 // same smooth branch of the renderer.
+// evals (optional): pixel-splat evaluations performed under the
+// early-termination rules (the raster's algorithmic VALU work).
 template <typename T>
 void raster_all(uint32_t W, uint32_t H, uint32_t tile, const uint32_t* ranges, const uint32_t* vals,
                 const T* conic, const T* xy, const T* rgb, T* out, T* tfinal, uint32_t* last,
-                uint64_t* sig = nullptr) {
+                uint64_t* sig = nullptr, uint64_t* evals = nullptr) {
     const uint32_t TXn = (W + tile - 1) / tile, TYn = (H + tile - 1) / tile;
     const int64_t ntile = (int64_t)TXn * TYn;
-#pragma omp parallel for schedule(dynamic, 1)
+    uint64_t ev = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : ev)
     for (int64_t t = 0; t < ntile; ++t) {
         const uint32_t tx = (uint32_t)(t % TXn), ty = (uint32_t)(t / TXn);
         const uint32_t start = ranges[t * 2], end = ranges[t * 2 + 1];
@@ -417,6 +420,7 @@ void raster_all(uint32_t W, uint32_t H, uint32_t tile, const uint32_t* ranges, c
                 uint32_t lastc = 0;
                 uint64_t h = 1469598103934665603ull;
                 for (uint32_t i = start; i < end; ++i) {
+                    ++ev;
                     const uint32_t g = vals[i];
                     const T A = conic[g * 4], B = conic[g * 4 + 1], C = conic[g * 4 + 2];
                     const T op = conic[g * 4 + 3];
@@ -449,12 +453,13 @@ void raster_all(uint32_t W, uint32_t H, uint32_t tile, const uint32_t* ranges, c
                 if (sig) sig[p] = h;
             }
     }
+    if (evals) *evals = ev;
 }
 
 template <typename T>
 int render(const float* packed, uint32_t n, const Cam& cam, uint32_t tile, uint64_t cap, T* out,
            T* tfinal, uint32_t* last, uint64_t* instances, double* stage_ms, int use_radix,
-           uint64_t* signature = nullptr) {
+           uint64_t* signature = nullptr, uint64_t* evals = nullptr) {
     using clk = std::chrono::steady_clock;
     auto ms = [](clk::time_point a, clk::time_point b) {
         return std::chrono::duration<double, std::milli>(b - a).count();
@@ -488,7 +493,7 @@ int render(const float* packed, uint32_t n, const Cam& cam, uint32_t tile, uint6
     auto t5 = clk::now();
     std::vector<uint64_t> psig(signature ? (size_t)W * H : 0);
     raster_all<T>(W, H, tile, ranges.data(), vals.data(), conic.data(), xy.data(), rgb.data(), out,
-                  tfinal, last, signature ? psig.data() : nullptr);
+                  tfinal, last, signature ? psig.data() : nullptr, evals);
     auto t6 = clk::now();
     if (signature) {
         uint64_t h = fnv(1469598103934665603ull, total);
@@ -991,9 +996,9 @@ void orc_raster_f64(uint32_t W, uint32_t H, uint32_t tile, const uint32_t* range
 
 int orc_render_f32(const float* packed, uint32_t n, const void* cam, uint32_t tile, uint64_t cap,
                    float* out, float* tfinal, uint32_t* last, uint64_t* instances, double* stage_ms,
-                   int use_radix) {
+                   int use_radix, uint64_t* evals) {
     return render<float>(packed, n, *(const Cam*)cam, tile, cap, out, tfinal, last, instances, stage_ms,
-                         use_radix);
+                         use_radix, nullptr, evals);
 }
 
 void orc_backward_f64(const float* packed, uint32_t n, const void* cam, uint32_t tile, const double* dLdC,

@@ -53,7 +53,7 @@ def lib():
                                      _f64p, _f64p, _f64p, _f64p, _f64p, _u32p]
         L.orc_render_f32.restype = ctypes.c_int
         L.orc_render_f32.argtypes = [_f32p, ctypes.c_uint32, _f32p, ctypes.c_uint32, ctypes.c_uint64,
-                                     _f32p, _f32p, _u32p, _u64p, _f64p, ctypes.c_int]
+                                     _f32p, _f32p, _u32p, _u64p, _f64p, ctypes.c_int, ctypes.c_void_p]
         L.orc_render_f64.restype = ctypes.c_int
         L.orc_render_f64.argtypes = [_f32p, ctypes.c_uint32, _f32p, ctypes.c_uint32, ctypes.c_uint64,
                                      _f64p, _f64p, _u32p, _u64p, ctypes.c_void_p]
@@ -139,7 +139,7 @@ def raster(width, height, tile, ranges, vals, conic, xy, rgb, fp64=False):
     return out.reshape(3, height, width), tf.reshape(height, width), last.reshape(height, width)
 
 
-def render(packed, n, uni, tile=16, cap=0, fp64=False, use_radix=False, signature=False):
+def render(packed, n, uni, tile=16, cap=0, fp64=False, use_radix=False, signature=False, evals=None):
     """Whole-frame oracle.  Returns (rgb[3,H,W], T_final[H,W], last[H,W], instances, stage_ms);
     with signature=True (FP64 only) the last item is instead the hash of every
     discrete decision of the frame."""
@@ -158,7 +158,10 @@ def render(packed, n, uni, tile=16, cap=0, fp64=False, use_radix=False, signatur
         if signature:
             ms = int(sig[0])
     else:
-        lib().orc_render_f32(packed, n, uni, tile, cap, out, tf, last, inst, ms, 1 if use_radix else 0)
+        ev = np.zeros(1, np.uint64)
+        lib().orc_render_f32(packed, n, uni, tile, cap, out, tf, last, inst, ms, 1 if use_radix else 0, ev.ctypes.data)
+        if evals is not None:
+            evals.append(int(ev[0]))
     return (out.reshape(3, H, W), tf.reshape(H, W), last.reshape(H, W), int(inst[0]), ms)
 
 

@@ -17,6 +17,7 @@
 // position; d/dA, d/dB, d/dC of the conic; d/d opacity; d/d(r, g, b); pad.
 #include <cstring>
 
+#include "cull.h"
 #include "gsr_internal.h"
 #include "wave.h"
 
@@ -88,11 +89,15 @@ __global__ __launch_bounds__(64) void k_raster_backward(
     const uint32_t start = r.x;
     const uint32_t stop = min(min(r.y, n_inst), start + maxlast);  // exclusive end of the replay
     if (stop <= start) return;
+    const float rxa = (float)(tx * TS), rxb = (float)min(tx * TS + TS - 1, W - 1);
+    const float rya = (float)(ty * TS), ryb = (float)min(ty * TS + TS - 1, H - 1);
     for (int64_t b = (int64_t)start + (int64_t)((stop - start - 1) / 64) * 64; b >= (int64_t)start; b -= 64) {
         const uint32_t idx = (uint32_t)b + lane;
         float4 co = make_float4(0.f, 0.f, 0.f, 0.f), cl = make_float4(0.f, 0.f, 0.f, 0.f);
         float2 p = make_float2(0.f, 0.f);
         uint32_t gid = 0xFFFFFFFFu;
+        bool live = false;
+        uint32_t bands = 0;
         if (idx < stop) {
             const uint32_t g = vals[idx];
             if (g < splats) {
@@ -100,10 +105,17 @@ __global__ __launch_bounds__(64) void k_raster_backward(
                 co = conic[g];
                 p = xy[g];
                 cl = rgb[g];
+                const float qmax = reach_qmax(co.w);
+                live = reaches_rect(p.x, p.y, co.x, co.y, co.z, qmax, rxa, rxb, rya, ryb);
+                if (live)
+                    bands = band_mask<PPL>(p.y, reach_half_height(co.x, co.y, co.z, qmax), rya, (float)STEP);
             }
         }
-        const int m = (int)min(64u, stop - (uint32_t)b);
-        for (int j = m - 1; j >= 0; --j) {
+        // live splats back to front (highest lane first)
+        for (uint64_t lm = ballot(live); lm != 0;) {
+            const int j = 63 - (int)__builtin_clzll(lm);
+            lm &= ~(1ull << j);
+            const uint32_t bm = (uint32_t)__builtin_amdgcn_readlane((int)bands, j);
             const uint32_t pos = (uint32_t)b - start + (uint32_t)j + 1;
             const float gx = bcastf(p.x, j), gy = bcastf(p.y, j);
             const float A = bcastf(co.x, j), B = bcastf(co.y, j), C = bcastf(co.z, j), op = bcastf(co.w, j);
@@ -114,6 +126,7 @@ __global__ __launch_bounds__(64) void k_raster_backward(
             bool any = false;
 #pragma unroll
             for (int k = 0; k < PPL; ++k) {
+                if (!((bm >> k) & 1u)) continue;
                 if (pos > lst[k]) continue;
                 const float dy = gy - fy[k];
                 const float q = __builtin_fmaf(ad, dx, (C * dy) * dy);

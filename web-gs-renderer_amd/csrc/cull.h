@@ -1,0 +1,69 @@
+// cull.h — conservative reach test of a splat against a tile and its row bands.
+//
+// A pixel receives a splat only if alpha = min(0.99, op * exp(power)) >= 1/255
+// with power = -q/2, q = A dx^2 + 2 B dx dy + C dy^2 (raseter.wgsl:80-95,
+// 113-117), i.e. only where q <= q_max = 2 ln(255 op).  The raster kernels
+// skip a splat for a whole tile when the minimum of q over the tile's pixel
+// rectangle exceeds q_max, and skip a band of pixel rows when the ellipse's
+// y-extent misses it.  The threshold is widened (x1.02 + 0.1) so FP32
+// rounding of q, of the logarithm and of the minimum can never cull a splat
+// some pixel would blend: skipping is exactly equivalent to evaluating.
+// Any NaN in the test leaves the splat live.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace gsr {
+
+// q over the edge with dx fixed, dy in [lo, hi]: minimum at dy = -B dx / C.
+__device__ __forceinline__ float edge_min_dx(float A, float B, float C, float dx, float lo, float hi) {
+    const float dy = fminf(fmaxf(-B * dx / C, lo), hi);
+    return A * dx * dx + 2.0f * B * dx * dy + C * dy * dy;
+}
+
+__device__ __forceinline__ float edge_min_dy(float A, float B, float C, float dy, float lo, float hi) {
+    const float dx = fminf(fmaxf(-B * dy / A, lo), hi);
+    return A * dx * dx + 2.0f * B * dx * dy + C * dy * dy;
+}
+
+// Widened reach threshold on q; negative when the opacity alone keeps alpha
+// below 1/255 everywhere.
+__device__ __forceinline__ float reach_qmax(float op) {
+    const float l = __logf(255.0f * op);   // v_log_f32 based; error absorbed by the widening
+    return l > 0.0f ? 1.02f * (2.0f * l) + 0.1f : -1.0f;
+}
+
+// True unless the splat provably misses every pixel centre of [xa, xb] x [ya, yb].
+__device__ __forceinline__ bool reaches_rect(float gx, float gy, float A, float B, float C, float qmax, float xa,
+                                             float xb, float ya, float yb) {
+    if (!(qmax >= 0.0f)) return false;
+    if (gx >= xa && gx <= xb && gy >= ya && gy <= yb) return true;
+    const float dxlo = gx - xb, dxhi = gx - xa, dylo = gy - yb, dyhi = gy - ya;
+    float q = edge_min_dx(A, B, C, dxhi, dylo, dyhi);
+    q = fminf(q, edge_min_dx(A, B, C, dxlo, dylo, dyhi));
+    q = fminf(q, edge_min_dy(A, B, C, dyhi, dxlo, dxhi));
+    q = fminf(q, edge_min_dy(A, B, C, dylo, dxlo, dxhi));
+    return !(q > qmax);
+}
+
+// Half-extent in y of the ellipse q <= qmax: sqrt(qmax * A / (A C - B^2)).
+__device__ __forceinline__ float reach_half_height(float A, float B, float C, float qmax) {
+    return sqrtf(qmax * A / (A * C - B * B));
+}
+
+// Bit k set when rows [y0 + k*step, y0 + k*step + step - 1] may be reached.
+template <int BANDS>
+__device__ __forceinline__ uint32_t band_mask(float gy, float ey, float y0, float step) {
+    uint32_t m = 0;
+#pragma unroll
+    for (int k = 0; k < BANDS; ++k) {
+        const float lo = y0 + k * step, hi = lo + step - 1.0f;
+        const bool miss = (gy + ey < lo) || (gy - ey > hi);
+        if (!miss) m |= 1u << k;
+    }
+    return m;
+}
+
+}  // namespace gsr

@@ -14,6 +14,10 @@
 //   rgb += colour * (alpha T); T = T'.
 // A power below -5.55 is skipped before the exp: exp(-5.55) < 1/255 and the
 // opacity is at most 1, so the reference's alpha test would reject it too.
+// Per batch each lane also tests its splat against the tile (cull.h): splats
+// that cannot reach any pixel of the tile, and row bands a splat's ellipse
+// misses, are skipped without changing any pixel's result.
+#include "cull.h"
 #include "gsr_internal.h"
 #include "wave.h"
 
@@ -77,6 +81,8 @@ __device__ __forceinline__ void raster_tile(uint32_t W, uint32_t H, uint32_t x0,
         c0[k] = c1[k] = c2[k] = 0.0f;
         lst[k] = 0;
     }
+    const float rxa = (float)x0, rxb = (float)min(x0 + TS - 1, W - 1);
+    const float rya = (float)y0, ryb = (float)min(y0 + TS - 1, H - 1);
     for (uint32_t b = start; b < end; b += 64) {
         bool active = false;
 #pragma unroll
@@ -85,16 +91,24 @@ __device__ __forceinline__ void raster_tile(uint32_t W, uint32_t H, uint32_t x0,
         const uint32_t idx = b + lane;
         float4 co = make_float4(0.f, 0.f, 0.f, 0.f), cl = make_float4(0.f, 0.f, 0.f, 0.f);
         float2 p = make_float2(0.f, 0.f);
+        bool live = false;
+        uint32_t bands = 0;
         if (idx < end) {
             const uint32_t g = vals[idx];
             if (g < splats) {
                 co = conic[g];
                 p = xy[g];
                 cl = rgb[g];
+                const float qmax = reach_qmax(co.w);
+                live = reaches_rect(p.x, p.y, co.x, co.y, co.z, qmax, rxa, rxb, rya, ryb);
+                if (live)
+                    bands = band_mask<PPL>(p.y, reach_half_height(co.x, co.y, co.z, qmax), (float)y0, (float)STEP);
             }
         }
-        const int m = (int)min(64u, end - b);
-        for (int j = 0; j < m; ++j) {
+        // Only splats that can reach the tile, in range order.
+        for (uint64_t lm = ballot(live); lm != 0; lm &= lm - 1) {
+            const int j = (int)__builtin_ctzll(lm);
+            const uint32_t bm = (uint32_t)__builtin_amdgcn_readlane((int)bands, j);
             const float gx = bcast(p.x, j), gy = bcast(p.y, j);
             const float A = bcast(co.x, j), B = bcast(co.y, j), C = bcast(co.z, j), op = bcast(co.w, j);
             const float cr = bcast(cl.x, j), cg = bcast(cl.y, j), cb = bcast(cl.z, j);
@@ -103,6 +117,7 @@ __device__ __forceinline__ void raster_tile(uint32_t W, uint32_t H, uint32_t x0,
             const uint32_t pos = b - start + (uint32_t)j + 1;
 #pragma unroll
             for (int k = 0; k < PPL; ++k) {
+                if (!((bm >> k) & 1u)) continue;
                 if (done[k]) continue;
                 const float dy = gy - fy[k];
                 const float q = __builtin_fmaf(ad, dx, (C * dy) * dy);
