@@ -874,9 +874,10 @@ void backward_f64(const float* packed, uint32_t n, const Cam& cam, uint32_t tile
 }
 
 // FP32 replay of the raster adjoint with the same steps as the GPU kernel
-// (T recovered by division, colour-behind recurrence), per-splat sums in
-// double.  A diagnostic twin of the kernel: it separates FP32 term error
-// from summation order.
+// (T recovered by dividing by 1 - alpha, colour-behind recurrence), per-splat
+// sums in double.  The kernel multiplies by a 1-ulp hardware reciprocal
+// instead of dividing; otherwise the arithmetic is the same, so the twin
+// separates FP32 term error from summation order.
 void raster_backward_f32(uint32_t W, uint32_t H, uint32_t tile, const uint32_t* ranges, const uint32_t* vals,
                          const float* conic, const float* xy, const float* rgb, const float* tfinal,
                          const uint32_t* last, const float* dLdC, uint32_t n, double* grad2d) {

@@ -6,7 +6,7 @@
 //   Each pixel replays its splats back to front, starting at its stored last
 //   contributor with its stored final transmittance; alphas are recomputed
 //   with the forward's exact FP32 steps (so the blend decisions are the
-//   forward's), T is recovered as T / (1 - alpha), and the colour behind each
+//   forward's), T is recovered as T * rcp(1 - alpha), and the colour behind each
 //   splat is accumulated on the way.  Per splat the 9 gradient terms of the
 //   lane's pixels are summed in registers, reduced across the wave with DPP,
 //   and one lane adds them to the splat's 2-D gradient with float atomics.
@@ -136,8 +136,8 @@ __global__ __launch_bounds__(64) void k_raster_backward(
                 const float oe = op * e;
                 const float alpha = fminf(kAlphaClamp, oe);
                 if (alpha < kAlphaMin) continue;
-                // blended in the forward: undo it
-                const float Tb = T[k] / (1.0f - alpha);
+                // blended in the forward: undo it (v_rcp_f32: 1 ulp, no IEEE division sequence)
+                const float Tb = T[k] * __builtin_amdgcn_rcpf(1.0f - alpha);
                 T[k] = Tb;
                 const float gal = ((cr - Br[k]) * gr[k] + (cg - Bg[k]) * gg[k] + (cb - Bb[k]) * gb[k]) * Tb;
                 const float w = alpha * Tb;

@@ -39,7 +39,7 @@ static int plan_batch(Context* c, uint32_t count, const gsr_camera* cams, uint32
     if (p->tile != 8 && p->tile != 16 && p->tile != 32)
         return fail(c, GSR_ERR_ARGUMENT, "tile size %u not in {8, 16, 32}", p->tile);
     p->raster_waves = (opts && opts->raster_waves_per_cu) ? opts->raster_waves_per_cu : 16;
-    p->sort_blocks = (opts && opts->sort_blocks_per_cu) ? opts->sort_blocks_per_cu : 2;
+    p->sort_blocks = (opts && opts->sort_blocks_per_cu) ? opts->sort_blocks_per_cu : 3;
     p->info.resize(views);
     p->pix.resize(views);
     uint64_t tiles = 0, floats = 0;

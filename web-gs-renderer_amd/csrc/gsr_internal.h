@@ -135,7 +135,7 @@ struct Context {
     int state_sorted_index = 0;
     uint32_t last_frame_w = 0, last_frame_h = 0;  // size of the frame in "image"
     // launch shape knobs (set per call from gsr_options)
-    uint32_t sort_blocks_per_cu = 2;
+    uint32_t sort_blocks_per_cu = 3;
     // prefix of the scan/sort scratch buffer names ("" plain frames, "b_" batches)
     const char* scratch_prefix = "";
     // batches

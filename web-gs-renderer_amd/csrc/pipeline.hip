@@ -47,7 +47,7 @@ int plan_frame(Context* c, uint32_t count, const gsr_camera* cam, const gsr_opti
         cap = kMaxSortKeys;
     }
     p->capacity = cap;
-    p->sort_blocks = (opts && opts->sort_blocks_per_cu) ? opts->sort_blocks_per_cu : 2;
+    p->sort_blocks = (opts && opts->sort_blocks_per_cu) ? opts->sort_blocks_per_cu : 3;
     return GSR_OK;
 }
 

@@ -17,6 +17,7 @@ constexpr int kScanThreads = 256;
 constexpr int kScanItems = 16;
 constexpr int kScanTile = kScanThreads * kScanItems;
 constexpr uint64_t kFlagAgg = 1ull << 62, kFlagInc = 2ull << 62, kValMask = (1ull << 62) - 1;
+constexpr int kLookBack = 16;
 
 // counts[0..n) -> out[0..n], out[n] = total.  status[0] is the tile ticket,
 // status[1 + t] the look-back word of tile t; all zeroed before the launch.
@@ -59,18 +60,31 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(const uint32_t* in, uint3
             st_release_u64(&status[0], kFlagInc | block_total);
         } else {
             st_release_u64(&status[tile], kFlagAgg | block_total);
+            // batched look-back (see sort.hip): kLookBack words per round trip
             int64_t j = (int64_t)tile - 1;
             uint32_t spins = 0;
-            while (j >= 0) {
-                const uint64_t s = ld_acquire_u64(&status[j]);
-                if ((s >> 62) == 0) {
-                    if (++spins > kSpinLimit) { atomicOr(&counters->error, 1u); break; }
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
+            bool found = false;
+            while (!found && j >= 0) {
+                uint64_t w[kLookBack];
+#pragma unroll
+                for (int d = 0; d < kLookBack; ++d) w[d] = (j - d >= 0) ? ld_acquire_u64(&status[j - d]) : kFlagInc;
+#pragma unroll
+                for (int d = 0; d < kLookBack; ++d) {
+                    if (found || j - d < 0) continue;
+                    uint64_t s = w[d];
+                    while ((s >> 62) == 0) {
+                        if (++spins > kSpinLimit) {
+                            atomicOr(&counters->error, 1u);
+                            s = kFlagInc;
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+                        s = ld_acquire_u64(&status[j - d]);
+                    }
+                    excl += s & kValMask;
+                    if (s & kFlagInc) found = true;
                 }
-                excl += s & kValMask;
-                if ((s & kFlagInc) != 0) break;
-                --j;
+                j -= kLookBack;
             }
             st_release_u64(&status[tile], kFlagInc | ((excl + block_total) & kValMask));
         }

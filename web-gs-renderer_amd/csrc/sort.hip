@@ -22,6 +22,7 @@ namespace gsr {
 namespace {
 
 constexpr uint32_t kAgg = 1u << 30, kInc = 2u << 30;
+constexpr int kLookBack = 16;  // predecessor words fetched per look-back step
 
 __device__ __forceinline__ uint32_t read_n(const uint32_t* n_dev, uint64_t n_host) {
     return n_dev ? *n_dev : (uint32_t)n_host;
@@ -156,18 +157,34 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(
         const uint32_t local = block_exclusive_256(cnt, s4, nullptr);
         uint32_t excl = 0;
         if (tile > 0) {
+            // Batched look-back: the words of up to kLookBack predecessors are
+            // loaded together (independent loads, one round trip), then
+            // consumed nearest first; only a word still not ready is re-polled.
             int64_t j = (int64_t)tile - 1;
             uint32_t spins = 0;
-            while (j >= 0) {
-                const uint32_t s = ld_acquire_u32(status + (size_t)j * kRadix + tid);
-                if ((s >> 30) == 0) {
-                    if (++spins > kSpinLimit) { atomicOr(error, 2u); break; }
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
+            bool found = false;
+            while (!found && j >= 0) {
+                uint32_t w[kLookBack];
+#pragma unroll
+                for (int d = 0; d < kLookBack; ++d)
+                    w[d] = (j - d >= 0) ? ld_acquire_u32(status + (size_t)(j - d) * kRadix + tid) : kInc;
+#pragma unroll
+                for (int d = 0; d < kLookBack; ++d) {
+                    if (found || j - d < 0) continue;
+                    uint32_t s = w[d];
+                    while ((s >> 30) == 0) {
+                        if (++spins > kSpinLimit) {
+                            atomicOr(error, 2u);
+                            s = kInc;
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+                        s = ld_acquire_u32(status + (size_t)(j - d) * kRadix + tid);
+                    }
+                    excl += s & kStatusValueMask;
+                    if (s & kInc) found = true;
                 }
-                excl += s & kStatusValueMask;
-                if (s & kInc) break;
-                --j;
+                j -= kLookBack;
             }
             st_release_u32(my_status, kInc | ((excl + cnt) & kStatusValueMask));
         }
