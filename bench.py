@@ -35,7 +35,7 @@ def log(msg):
 
 def algorithmic_bytes(stage: str, n_splats: int, instances: int, visible: int, width: int, height: int,
                       tiles: int, key_bytes: int, sort_passes: int) -> float:
-    """Algorithmic HBM bytes per launch of each stage."""
+    """Algorithmic HBM bytes per launch of each stage: what the stage must read and write at least."""
     if stage == "preprocess":
         # read 236 B of parameters for a visible splat (56 B when culled before SH),
         # write tiles 4 + meta 16 + conic 16 + xy 8 + rgb 16
@@ -131,7 +131,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the renderer has no "
-              f"multi-GPU path in this build.", file=sys.stderr)
+              f"multi-GPU path.", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

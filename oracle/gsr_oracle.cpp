@@ -1,7 +1,7 @@
 // gsr_oracle.cpp — CPU restatement of the reference splat pass graph.
 //
-// TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
-// renderer: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+// TEST INFRASTRUCTURE ONLY.  This library is the CPU reference the HIP
+// renderer is checked against: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
 // leg load it.  The product never calls it.
 //
 // It restates, in FP32 and FP64, the per-frame pipeline of

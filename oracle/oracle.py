@@ -1,7 +1,7 @@
 """ctypes access to the CPU oracle (liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
-bench.py's cpu_baseline leg as the checker.  The renderer never imports it.
+bench.py's cpu_baseline leg, as the reference for checks and CPU timing.  The renderer never imports it.
 """
 from __future__ import annotations
 

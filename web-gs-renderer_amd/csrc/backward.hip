@@ -2,14 +2,15 @@
 // this differentiates src/shaders/raseter.wgsl:35-117 and
 // src/shaders/preprocess.wgsl:58-284).
 //
-// k_raster_backward: one wave64 per tile with the forward's pixel layout.
-//   Each pixel replays its splats back to front, starting at its stored last
-//   contributor with its stored final transmittance; alphas are recomputed
-//   with the forward's exact FP32 steps (so the blend decisions are the
-//   forward's), T is recovered as T * rcp(1 - alpha), and the colour behind each
-//   splat is accumulated on the way.  Per splat the 9 gradient terms of the
-//   lane's pixels are summed in registers, reduced across the wave with DPP,
-//   and one lane adds them to the splat's 2-D gradient with float atomics.
+// k_raster_backward: one wave64 per 8x8 pixel block of a tile, one pixel per
+//   lane, as in the forward.  Each pixel replays its splats back to front,
+//   starting at its stored last contributor with its stored final
+//   transmittance; alphas are recomputed with the forward's exact FP32 steps
+//   (so the blend decisions are the forward's), T is recovered as
+//   T * rcp(1 - alpha), and the colour behind each splat is accumulated on the
+//   way.  Per splat the 9 gradient terms are summed over the wave (8 by a
+//   swap-and-add transposition, 1 by DPP) and added to the splat's 2-D
+//   gradient with float atomics, one instruction for the 8.
 // k_preprocess_backward: one thread per splat, chains the 2-D gradient
 //   (pixel position, conic, opacity, colour) back to the 59 parameters.
 //
@@ -45,59 +46,103 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
     return v;
 }
 
+// Sums of 8 per-lane values over the wave by pairwise transposition.  A
+// permlane32 swap exchanges the upper half of one register with the lower
+// half of another, so the sum of the pair's two results holds, in one half
+// of the wave, half-sums of the first value and in the other half those of
+// the second: 4 swaps and 4 adds fold 8 values into 4.  permlane16 swaps fold
+// 4 into 2 the same way on 16-lane rows; a DPP rotate by 8 folds 2 into 1 on
+// 8-lane groups; three DPP steps (half-row mirror, quad swaps) finish each
+// group.  Lane 8i then holds one value's wave total.  Which value lands where
+// depends only on the swap network, so the same network run once on the
+// values' indices (fold_tags) names it.
+struct Fold8 {
+    float sum;     // wave total of value tag, valid in lanes 8i
+    uint32_t tag;  // index of that value, 0..7
+};
+
+__device__ __forceinline__ float fold32(float d, float s) {
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(d), __float_as_uint(s), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+__device__ __forceinline__ float fold16(float d, float s) {
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(d), __float_as_uint(s), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+__device__ __forceinline__ uint32_t tag32(uint32_t d, uint32_t s) {
+    return __builtin_amdgcn_permlane32_swap(d, s, false, false)[0];
+}
+__device__ __forceinline__ uint32_t tag16(uint32_t d, uint32_t s) {
+    return __builtin_amdgcn_permlane16_swap(d, s, false, false)[0];
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_f32(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+__device__ __forceinline__ uint32_t fold_tags() {
+    const uint32_t u0 = tag32(0, 1), u1 = tag32(2, 3), u2 = tag32(4, 5), u3 = tag32(6, 7);
+    const uint32_t w0 = tag16(u0, u1), w1 = tag16(u2, u3);
+    return (threadIdx.x & 8) ? w1 : w0;
+}
+
+__device__ __forceinline__ float fold8(const float (&v)[8]) {
+    const float u0 = fold32(v[0], v[1]), u1 = fold32(v[2], v[3]);
+    const float u2 = fold32(v[4], v[5]), u3 = fold32(v[6], v[7]);
+    const float w0 = fold16(u0, u1), w1 = fold16(u2, u3);
+    const bool hi = (threadIdx.x & 8) != 0;
+    float z = (hi ? w1 : w0) + dpp_f32<0x128>(hi ? w0 : w1);  // row_ror:8
+    z += dpp_f32<0x141>(z);                                     // row_half_mirror
+    z += dpp_f32<0xB1>(z);                                      // quad_perm [1,0,3,2]
+    z += dpp_f32<0x4E>(z);                                      // quad_perm [2,3,0,1]
+    return z;
+}
+
+// One 8x8 block of one tile per wave64 (one pixel per lane); workgroups of 4
+// waves, workgroup ids remapped per XCD as in the forward.
 template <int TS>
-__global__ __launch_bounds__(64) void k_raster_backward(
-    uint32_t W, uint32_t H, uint32_t tiles_x, uint32_t splats, const uint32_t* n_dev,
+__global__ __launch_bounds__(256) void k_raster_backward(
+    uint32_t W, uint32_t H, uint32_t tiles_x, uint32_t num_tiles, uint32_t splats, const uint32_t* n_dev,
     const uint2* __restrict__ ranges, const uint32_t* __restrict__ vals, const float4* __restrict__ conic,
     const float2* __restrict__ xy, const float4* __restrict__ rgb, const float* __restrict__ tfinal,
     const uint32_t* __restrict__ last, const float* __restrict__ dLdC, float* __restrict__ grad2d) {
-    constexpr int PPL = TS * TS / 64;
-    constexpr int STEP = 64 / TS;
-    const uint32_t tile = blockIdx.x;
+    constexpr uint32_t SIDE = TS / 8, BLOCKS = SIDE * SIDE;
+    const uint32_t block = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t wv = block * 4 + (threadIdx.x >> 6);
+    const uint32_t tile = wv / BLOCKS, sub = wv % BLOCKS;
+    if (tile >= num_tiles) return;
+    const uint32_t lane = threadIdx.x & 63;
     const uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
-    const uint32_t lane = threadIdx.x;
-    const uint32_t x = tx * TS + lane % TS;
-    const uint32_t y0 = ty * TS + lane / TS;
-    const float fx = (float)x;
+    const uint32_t x0 = tx * TS + (sub % SIDE) * 8, y0 = ty * TS + (sub / SIDE) * 8;
+    const uint32_t x = x0 + lane % 8, y = y0 + lane / 8;
+    const float fx = (float)x, fy = (float)y;
     const size_t plane = (size_t)W * H;
-    float fy[PPL], T[PPL], gr[PPL], gg[PPL], gb[PPL], Br[PPL], Bg[PPL], Bb[PPL];
-    uint32_t lst[PPL];
-    uint32_t lane_max = 0;
-#pragma unroll
-    for (int k = 0; k < PPL; ++k) {
-        const uint32_t y = y0 + k * STEP;
-        fy[k] = (float)y;
-        Br[k] = Bg[k] = Bb[k] = 0.0f;
-        if (x < W && y < H) {
-            const size_t o = (size_t)y * W + x;
-            T[k] = tfinal[o];
-            lst[k] = last[o];
-            gr[k] = dLdC[o];
-            gg[k] = dLdC[plane + o];
-            gb[k] = dLdC[2 * plane + o];
-        } else {
-            T[k] = 1.0f;
-            lst[k] = 0;
-            gr[k] = gg[k] = gb[k] = 0.0f;
-        }
-        lane_max = max(lane_max, lst[k]);
+    float T = 1.0f, gr = 0.0f, gg = 0.0f, gb = 0.0f, Br = 0.0f, Bg = 0.0f, Bb = 0.0f;
+    uint32_t lst = 0;
+    if (x < W && y < H) {
+        const size_t o = (size_t)y * W + x;
+        T = tfinal[o];
+        lst = last[o];
+        gr = dLdC[o];
+        gg = dLdC[plane + o];
+        gb = dLdC[2 * plane + o];
     }
-    const uint32_t maxlast = wave_max_u32(lane_max);
+    const uint32_t maxlast = wave_max_u32(lst);
     if (maxlast == 0) return;
     const uint2 r = ranges[tile];
     const uint32_t n_inst = *n_dev;
     const uint32_t start = r.x;
     const uint32_t stop = min(min(r.y, n_inst), start + maxlast);  // exclusive end of the replay
     if (stop <= start) return;
-    const float rxa = (float)(tx * TS), rxb = (float)min(tx * TS + TS - 1, W - 1);
-    const float rya = (float)(ty * TS), ryb = (float)min(ty * TS + TS - 1, H - 1);
+    const uint32_t tag = fold_tags();
+    const float rxa = (float)x0, rxb = (float)min(x0 + 7, W - 1);
+    const float rya = (float)y0, ryb = (float)min(y0 + 7, H - 1);
     for (int64_t b = (int64_t)start + (int64_t)((stop - start - 1) / 64) * 64; b >= (int64_t)start; b -= 64) {
         const uint32_t idx = (uint32_t)b + lane;
         float4 co = make_float4(0.f, 0.f, 0.f, 0.f), cl = make_float4(0.f, 0.f, 0.f, 0.f);
         float2 p = make_float2(0.f, 0.f);
         uint32_t gid = 0xFFFFFFFFu;
         bool live = false;
-        uint32_t bands = 0;
         if (idx < stop) {
             const uint32_t g = vals[idx];
             if (g < splats) {
@@ -105,82 +150,62 @@ __global__ __launch_bounds__(64) void k_raster_backward(
                 co = conic[g];
                 p = xy[g];
                 cl = rgb[g];
-                const float qmax = reach_qmax(co.w);
-                live = reaches_rect(p.x, p.y, co.x, co.y, co.z, qmax, rxa, rxb, rya, ryb);
-                if (live)
-                    bands = band_mask<PPL>(p.y, reach_half_height(co.x, co.y, co.z, qmax), rya, (float)STEP);
+                live = reaches_rect(p.x, p.y, co.x, co.y, co.z, reach_qmax(co.w), rxa, rxb, rya, ryb);
             }
         }
         // live splats back to front (highest lane first)
         for (uint64_t lm = ballot(live); lm != 0;) {
             const int j = 63 - (int)__builtin_clzll(lm);
             lm &= ~(1ull << j);
-            const uint32_t bm = (uint32_t)__builtin_amdgcn_readlane((int)bands, j);
             const uint32_t pos = (uint32_t)b - start + (uint32_t)j + 1;
             const float gx = bcastf(p.x, j), gy = bcastf(p.y, j);
             const float A = bcastf(co.x, j), B = bcastf(co.y, j), C = bcastf(co.z, j), op = bcastf(co.w, j);
             const float cr = bcastf(cl.x, j), cg = bcastf(cl.y, j), cb = bcastf(cl.z, j);
-            const float dx = gx - fx;
-            const float ad = A * dx, bd = B * dx;
-            float a_x = 0.f, a_y = 0.f, a_A = 0.f, a_B = 0.f, a_C = 0.f, a_op = 0.f, a_r = 0.f, a_g = 0.f, a_b = 0.f;
+            float a[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};  // x, y, A, B, C, opacity, r, g
+            float a_b = 0.f;
             bool any = false;
-#pragma unroll
-            for (int k = 0; k < PPL; ++k) {
-                if (!((bm >> k) & 1u)) continue;
-                if (pos > lst[k]) continue;
-                const float dy = gy - fy[k];
-                const float q = __builtin_fmaf(ad, dx, (C * dy) * dy);
-                const float power = __builtin_fmaf(-0.5f, q, -(bd * dy));
-                if (power > 0.0f || power < -5.55f) continue;
-                const float e = exp_core(power);
-                const float oe = op * e;
-                const float alpha = fminf(kAlphaClamp, oe);
-                if (alpha < kAlphaMin) continue;
-                // blended in the forward: undo it (v_rcp_f32: 1 ulp, no IEEE division sequence)
-                const float Tb = T[k] * __builtin_amdgcn_rcpf(1.0f - alpha);
-                T[k] = Tb;
-                const float gal = ((cr - Br[k]) * gr[k] + (cg - Bg[k]) * gg[k] + (cb - Bb[k]) * gb[k]) * Tb;
-                const float w = alpha * Tb;
-                a_r += w * gr[k];
-                a_g += w * gg[k];
-                a_b += w * gb[k];
-                if (oe <= kAlphaClamp) {
-                    const float gpow = alpha * gal;
-                    a_op += e * gal;
-                    a_A += gpow * (-0.5f * dx * dx);
-                    a_B += gpow * (-dx * dy);
-                    a_C += gpow * (-0.5f * dy * dy);
-                    a_x += gpow * (-(A * dx + B * dy));
-                    a_y += gpow * (-(C * dy + B * dx));
+            if (pos <= lst) {
+                const float dx = gx - fx, dy = gy - fy;
+                const float q = __builtin_fmaf(A * dx, dx, (C * dy) * dy);
+                const float power = __builtin_fmaf(-0.5f, q, -((B * dx) * dy));
+                if (!(power > 0.0f || power < -5.55f)) {
+                    const float e = exp_core(power);
+                    const float oe = op * e;
+                    const float alpha = fminf(kAlphaClamp, oe);
+                    if (alpha >= kAlphaMin) {
+                        // blended in the forward: undo it (v_rcp_f32: 1 ulp, no IEEE division sequence)
+                        const float Tb = T * __builtin_amdgcn_rcpf(1.0f - alpha);
+                        T = Tb;
+                        const float gal = ((cr - Br) * gr + (cg - Bg) * gg + (cb - Bb) * gb) * Tb;
+                        const float w = alpha * Tb;
+                        a[6] = w * gr;
+                        a[7] = w * gg;
+                        a_b = w * gb;
+                        if (oe <= kAlphaClamp) {
+                            const float gpow = alpha * gal;
+                            a[5] = e * gal;
+                            a[2] = gpow * (-0.5f * dx * dx);
+                            a[3] = gpow * (-dx * dy);
+                            a[4] = gpow * (-0.5f * dy * dy);
+                            a[0] = gpow * (-(A * dx + B * dy));
+                            a[1] = gpow * (-(C * dy + B * dx));
+                        }
+                        const float om = 1.0f - alpha;
+                        Br = alpha * cr + om * Br;
+                        Bg = alpha * cg + om * Bg;
+                        Bb = alpha * cb + om * Bb;
+                        any = true;
+                    }
                 }
-                const float om = 1.0f - alpha;
-                Br[k] = alpha * cr + om * Br[k];
-                Bg[k] = alpha * cg + om * Bg[k];
-                Bb[k] = alpha * cb + om * Bb[k];
-                any = true;
             }
             if (ballot(any) == 0) continue;
-            a_x = wave_sum_f32(a_x);
-            a_y = wave_sum_f32(a_y);
-            a_A = wave_sum_f32(a_A);
-            a_B = wave_sum_f32(a_B);
-            a_C = wave_sum_f32(a_C);
-            a_op = wave_sum_f32(a_op);
-            a_r = wave_sum_f32(a_r);
-            a_g = wave_sum_f32(a_g);
-            a_b = wave_sum_f32(a_b);
+            const float s8 = fold8(a);
+            const float sb = wave_sum_f32(a_b);
             const uint32_t g = (uint32_t)__builtin_amdgcn_readlane((int)gid, j);
-            if (lane == 63 && g < splats) {
+            if (g < splats) {
                 float* dst = grad2d + (size_t)g * 12;
-                unsafeAtomicAdd(dst + 0, a_x);
-                unsafeAtomicAdd(dst + 1, a_y);
-                unsafeAtomicAdd(dst + 2, a_A);
-                unsafeAtomicAdd(dst + 3, a_B);
-                unsafeAtomicAdd(dst + 4, a_C);
-                unsafeAtomicAdd(dst + 5, a_op);
-                unsafeAtomicAdd(dst + 6, a_r);
-                unsafeAtomicAdd(dst + 7, a_g);
-                unsafeAtomicAdd(dst + 8, a_b);
+                if ((lane & 7) == 0) unsafeAtomicAdd(dst + tag, s8);
+                if (lane == 63) unsafeAtomicAdd(dst + 8, sb);
             }
         }
     }
@@ -416,17 +441,18 @@ int launch_backward(Context* c, const float* params, uint32_t count, const CamK&
     const FrameCounters* counters = (const FrameCounters*)buffer(c, "counters", 0);
     mark(c, "backward raster");
     const uint32_t tiles = tx * ty;
+    const uint32_t blocks = ceil_div(tiles * (tile * tile / 64), 4u);
     switch (tile) {
         case 8:
-            hipLaunchKernelGGL(k_raster_backward<8>, dim3(tiles), dim3(64), 0, c->stream, W, H, tx, count,
+            hipLaunchKernelGGL(k_raster_backward<8>, dim3(blocks), dim3(256), 0, c->stream, W, H, tx, tiles, count,
                                &counters->n, ranges, vals, conic, xy, rgb, tfinal, last, dLdC, g2);
             break;
         case 16:
-            hipLaunchKernelGGL(k_raster_backward<16>, dim3(tiles), dim3(64), 0, c->stream, W, H, tx, count,
+            hipLaunchKernelGGL(k_raster_backward<16>, dim3(blocks), dim3(256), 0, c->stream, W, H, tx, tiles, count,
                                &counters->n, ranges, vals, conic, xy, rgb, tfinal, last, dLdC, g2);
             break;
         case 32:
-            hipLaunchKernelGGL(k_raster_backward<32>, dim3(tiles), dim3(64), 0, c->stream, W, H, tx, count,
+            hipLaunchKernelGGL(k_raster_backward<32>, dim3(blocks), dim3(256), 0, c->stream, W, H, tx, tiles, count,
                                &counters->n, ranges, vals, conic, xy, rgb, tfinal, last, dLdC, g2);
             break;
         default:

@@ -1,13 +1,13 @@
-// cull.h — conservative reach test of a splat against a tile and its row bands.
+// cull.h — conservative reach test of a splat against a pixel rectangle.
 //
 // A pixel receives a splat only if alpha = min(0.99, op * exp(power)) >= 1/255
 // with power = -q/2, q = A dx^2 + 2 B dx dy + C dy^2 (raseter.wgsl:80-95,
 // 113-117), i.e. only where q <= q_max = 2 ln(255 op).  The raster kernels
-// skip a splat for a whole tile when the minimum of q over the tile's pixel
-// rectangle exceeds q_max, and skip a band of pixel rows when the ellipse's
-// y-extent misses it.  The threshold is widened (x1.02 + 0.1) so FP32
-// rounding of q, of the logarithm and of the minimum can never cull a splat
-// some pixel would blend: skipping is exactly equivalent to evaluating.
+// skip a splat for a whole 8x8 pixel block when the minimum of q over the
+// block's pixel rectangle exceeds q_max.  The threshold is widened
+// (x1.02 + 0.1) so FP32 rounding of q, of the logarithm and of the minimum
+// can never cull a splat some pixel would blend: skipping is exactly
+// equivalent to evaluating.
 // Any NaN in the test leaves the splat live.
 #pragma once
 
@@ -46,24 +46,6 @@ __device__ __forceinline__ bool reaches_rect(float gx, float gy, float A, float 
     q = fminf(q, edge_min_dy(A, B, C, dyhi, dxlo, dxhi));
     q = fminf(q, edge_min_dy(A, B, C, dylo, dxlo, dxhi));
     return !(q > qmax);
-}
-
-// Half-extent in y of the ellipse q <= qmax: sqrt(qmax * A / (A C - B^2)).
-__device__ __forceinline__ float reach_half_height(float A, float B, float C, float qmax) {
-    return sqrtf(qmax * A / (A * C - B * B));
-}
-
-// Bit k set when rows [y0 + k*step, y0 + k*step + step - 1] may be reached.
-template <int BANDS>
-__device__ __forceinline__ uint32_t band_mask(float gy, float ey, float y0, float step) {
-    uint32_t m = 0;
-#pragma unroll
-    for (int k = 0; k < BANDS; ++k) {
-        const float lo = y0 + k * step, hi = lo + step - 1.0f;
-        const bool miss = (gy + ey < lo) || (gy - ey > hi);
-        if (!miss) m |= 1u << k;
-    }
-    return m;
 }
 
 }  // namespace gsr

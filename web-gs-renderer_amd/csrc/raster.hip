@@ -1,22 +1,23 @@
 // raster.hip — tile ranges (src/shaders/tile_ranges.wgsl:7-35) and the
 // front-to-back alpha-blend rasterizer (src/shaders/raseter.wgsl:35-117).
 //
-// Raster layout: one wave64 per tile.  Lane l owns column l % TS and rows
-// l / TS + k * (64 / TS) (1, 4 or 16 pixels for 8, 16, 32-pixel tiles), so a
-// splat's x offset is shared by a lane's pixels.  Each batch of 64 instances
-// is gathered one per lane (value -> conic/opacity, pixel, colour) and
-// broadcast to the wave with v_readlane (no LDS, no barriers); the batch loop
-// stops once every pixel of the tile is saturated, as the reference's
-// per-batch vote does.  Per pixel the FP32 steps are the oracle's:
+// Raster layout: one wave64 per 8x8 pixel block of a tile (1, 4 or 16 waves
+// for 8, 16, 32-pixel tiles), one pixel per lane; every wave of a tile walks
+// the tile's whole instance range, so each pixel sees the reference's
+// per-tile order.  Each batch of 64 instances is gathered one per lane
+// (value -> conic/opacity, pixel, colour) and broadcast to the wave with
+// v_readlane (no LDS, no barriers); the batch loop stops once every pixel of
+// the block is saturated, as the reference's per-batch vote does per tile.
+// Per pixel the FP32 steps are the oracle's:
 //   power = fma(-0.5, fma(A dx, dx, (C dy) dy), -(B dx) dy); skip if > 0;
 //   alpha = min(0.99, opacity * exp(power)); skip if < 1/255;
 //   T' = T (1 - alpha); stop before blending when T' < 1e-4;
 //   rgb += colour * (alpha T); T = T'.
 // A power below -5.55 is skipped before the exp: exp(-5.55) < 1/255 and the
 // opacity is at most 1, so the reference's alpha test would reject it too.
-// Per batch each lane also tests its splat against the tile (cull.h): splats
-// that cannot reach any pixel of the tile, and row bands a splat's ellipse
-// misses, are skipped without changing any pixel's result.
+// Per batch each lane also tests its splat against the 8x8 block (cull.h):
+// splats that cannot reach any pixel of the block are skipped without
+// changing any pixel's result.
 #include "cull.h"
 #include "gsr_internal.h"
 #include "wave.h"
@@ -55,140 +56,132 @@ __device__ __forceinline__ float bcast(float v, int lane) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
-// One tile of one view, run by one wave64.  (x0, y0) is the tile's first
-// pixel; out/tfinal/last address the view's image ([3][H][W] planes).
-template <int TS>
-__device__ __forceinline__ void raster_tile(uint32_t W, uint32_t H, uint32_t x0, uint32_t y0, uint32_t start,
+// One 8x8 block of one tile of one view, run by one wave64, one pixel per
+// lane: lane l owns pixel (x0 + l % 8, y0 + l / 8).  start/end delimit the
+// tile's instances; out/tfinal/last address the view's image ([3][H][W]).
+__device__ __forceinline__ void raster_block(uint32_t W, uint32_t H, uint32_t x0, uint32_t y0, uint32_t start,
                                             uint32_t end, uint32_t splats, const uint32_t* __restrict__ vals,
                                             const float4* __restrict__ conic, const float2* __restrict__ xy,
                                             const float4* __restrict__ rgb, float* __restrict__ out,
                                             float* __restrict__ tfinal, uint32_t* __restrict__ last) {
-    constexpr int PPL = TS * TS / 64;
-    constexpr int STEP = 64 / TS;
     const uint32_t lane = threadIdx.x & 63;
-    const uint32_t x = x0 + lane % TS;
-    const uint32_t yb = y0 + lane / TS;
-    const float fx = (float)x;
-    float fy[PPL], T[PPL], c0[PPL], c1[PPL], c2[PPL];
-    uint32_t lst[PPL];
-    bool done[PPL];
-#pragma unroll
-    for (int k = 0; k < PPL; ++k) {
-        const uint32_t y = yb + k * STEP;
-        done[k] = !(x < W && y < H);
-        fy[k] = (float)y;
-        T[k] = 1.0f;
-        c0[k] = c1[k] = c2[k] = 0.0f;
-        lst[k] = 0;
-    }
-    const float rxa = (float)x0, rxb = (float)min(x0 + TS - 1, W - 1);
-    const float rya = (float)y0, ryb = (float)min(y0 + TS - 1, H - 1);
+    const uint32_t x = x0 + lane % 8, y = y0 + lane / 8;
+    const bool inside = x < W && y < H;
+    bool done = !inside;
+    const float fx = (float)x, fy = (float)y;
+    float T = 1.0f, c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
+    uint32_t lst = 0;
+    const float rxa = (float)x0, rxb = (float)min(x0 + 7, W - 1);
+    const float rya = (float)y0, ryb = (float)min(y0 + 7, H - 1);
     for (uint32_t b = start; b < end; b += 64) {
-        bool active = false;
-#pragma unroll
-        for (int k = 0; k < PPL; ++k) active |= !done[k];
-        if (ballot(active) == 0) break;
+        if (ballot(!done) == 0) break;
         const uint32_t idx = b + lane;
         float4 co = make_float4(0.f, 0.f, 0.f, 0.f), cl = make_float4(0.f, 0.f, 0.f, 0.f);
         float2 p = make_float2(0.f, 0.f);
         bool live = false;
-        uint32_t bands = 0;
         if (idx < end) {
             const uint32_t g = vals[idx];
             if (g < splats) {
                 co = conic[g];
                 p = xy[g];
                 cl = rgb[g];
-                const float qmax = reach_qmax(co.w);
-                live = reaches_rect(p.x, p.y, co.x, co.y, co.z, qmax, rxa, rxb, rya, ryb);
-                if (live)
-                    bands = band_mask<PPL>(p.y, reach_half_height(co.x, co.y, co.z, qmax), (float)y0, (float)STEP);
+                live = reaches_rect(p.x, p.y, co.x, co.y, co.z, reach_qmax(co.w), rxa, rxb, rya, ryb);
             }
         }
-        // Only splats that can reach the tile, in range order.
+        // Only splats that can reach the block, in range order.
         for (uint64_t lm = ballot(live); lm != 0; lm &= lm - 1) {
             const int j = (int)__builtin_ctzll(lm);
-            const uint32_t bm = (uint32_t)__builtin_amdgcn_readlane((int)bands, j);
             const float gx = bcast(p.x, j), gy = bcast(p.y, j);
             const float A = bcast(co.x, j), B = bcast(co.y, j), C = bcast(co.z, j), op = bcast(co.w, j);
             const float cr = bcast(cl.x, j), cg = bcast(cl.y, j), cb = bcast(cl.z, j);
-            const float dx = gx - fx;
-            const float ad = A * dx, bd = B * dx;
-            const uint32_t pos = b - start + (uint32_t)j + 1;
-#pragma unroll
-            for (int k = 0; k < PPL; ++k) {
-                if (!((bm >> k) & 1u)) continue;
-                if (done[k]) continue;
-                const float dy = gy - fy[k];
-                const float q = __builtin_fmaf(ad, dx, (C * dy) * dy);
-                const float power = __builtin_fmaf(-0.5f, q, -(bd * dy));
-                if (power > 0.0f || power < -5.55f) continue;
-                const float alpha = fminf(kAlphaClamp, op * exp_core(power));
-                if (alpha < kAlphaMin) continue;
-                const float tn = T[k] * (1.0f - alpha);
-                if (tn < kTransmitMin) {
-                    done[k] = true;
-                    continue;
-                }
-                const float w = alpha * T[k];
-                c0[k] = __builtin_fmaf(cr, w, c0[k]);
-                c1[k] = __builtin_fmaf(cg, w, c1[k]);
-                c2[k] = __builtin_fmaf(cb, w, c2[k]);
-                T[k] = tn;
-                lst[k] = pos;
+            if (done) continue;
+            const float dx = gx - fx, dy = gy - fy;
+            const float q = __builtin_fmaf(A * dx, dx, (C * dy) * dy);
+            const float power = __builtin_fmaf(-0.5f, q, -((B * dx) * dy));
+            if (power > 0.0f || power < -5.55f) continue;
+            const float alpha = fminf(kAlphaClamp, op * exp_core(power));
+            if (alpha < kAlphaMin) continue;
+            const float tn = T * (1.0f - alpha);
+            if (tn < kTransmitMin) {
+                done = true;
+                continue;
             }
+            const float w = alpha * T;
+            c0 = __builtin_fmaf(cr, w, c0);
+            c1 = __builtin_fmaf(cg, w, c1);
+            c2 = __builtin_fmaf(cb, w, c2);
+            T = tn;
+            lst = b - start + (uint32_t)j + 1;
         }
     }
-    const size_t plane = (size_t)W * H;
-#pragma unroll
-    for (int k = 0; k < PPL; ++k) {
-        const uint32_t y = yb + k * STEP;
-        if (x < W && y < H) {
-            const size_t o = (size_t)y * W + x;
-            out[o] = c0[k];
-            out[plane + o] = c1[k];
-            out[2 * plane + o] = c2[k];
-            if (tfinal) tfinal[o] = T[k];
-            if (last) last[o] = lst[k];
-        }
+    if (inside) {
+        const size_t plane = (size_t)W * H;
+        const size_t o = (size_t)y * W + x;
+        out[o] = c0;
+        out[plane + o] = c1;
+        out[2 * plane + o] = c2;
+        if (tfinal) tfinal[o] = T;
+        if (last) last[o] = lst;
     }
 }
 
+// Workgroups of 4 waves.  Wave w of the grid takes 8x8 block w % BLOCKS of
+// tile w / BLOCKS (blocks of a tile in row order), so a tile's blocks (and,
+// for 8-pixel tiles, neighbouring tiles) share a CU.  An 8x8 block is the
+// most compact 64-pixel footprint, which culls the most splats per wave.
+// Workgroups reach the 8 XCDs round-robin; workgroup ids are remapped so that
+// each XCD rasterizes one contiguous run of tiles and the splats they share
+// stay in its L2.
 template <int TS>
-__global__ __launch_bounds__(64) void k_raster(uint32_t W, uint32_t H, uint32_t tiles_x, uint32_t splats,
-                                               const uint32_t* n_dev, uint32_t n_host,
-                                               const uint2* __restrict__ ranges,
-                                               const uint32_t* __restrict__ vals,
-                                               const float4* __restrict__ conic,
-                                               const float2* __restrict__ xy,
-                                               const float4* __restrict__ rgb, float* __restrict__ out,
-                                               float* __restrict__ tfinal, uint32_t* __restrict__ last) {
-    const uint32_t tile = blockIdx.x;
+__global__ __launch_bounds__(256) void k_raster(uint32_t W, uint32_t H, uint32_t tiles_x, uint32_t num_tiles,
+                                                uint32_t splats, const uint32_t* n_dev, uint32_t n_host,
+                                                const uint2* __restrict__ ranges,
+                                                const uint32_t* __restrict__ vals,
+                                                const float4* __restrict__ conic,
+                                                const float2* __restrict__ xy,
+                                                const float4* __restrict__ rgb, float* __restrict__ out,
+                                                float* __restrict__ tfinal, uint32_t* __restrict__ last) {
+    constexpr uint32_t SIDE = TS / 8, BLOCKS = SIDE * SIDE;
+    const uint32_t block = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t wv = block * 4 + (threadIdx.x >> 6);
+    const uint32_t tile = wv / BLOCKS, sub = wv % BLOCKS;
+    if (tile >= num_tiles) return;
     const uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
     const uint2 r = ranges[tile];
     const uint32_t n_inst = n_dev ? *n_dev : n_host;
     const uint32_t end = min(r.y, n_inst);  // ranges never exceed n; bounds the reads
-    raster_tile<TS>(W, H, tx * TS, ty * TS, r.x, end, splats, vals, conic, xy, rgb, out, tfinal, last);
+    raster_block(W, H, tx * TS + (sub % SIDE) * 8, ty * TS + (sub / SIDE) * 8, r.x, end, splats, vals, conic, xy, rgb,
+                 out, tfinal, last);
 }
 
-// Persistent form for view batches: each wave pulls global tile ids from a
-// ticket counter until all are done.  views[v] = {W, H, tiles_x, tile_base}
-// and pix[v] the float offset of view v's image.
+// Persistent form for view batches.  The work items are the 8x8 blocks of
+// every (view, tile), tile-major; a workgroup of 4 waves takes 4 consecutive
+// items per ticket (one 16-pixel tile, four 8-pixel tiles or a quarter of a
+// 32-pixel tile), one per wave, until all are done.  views[v] = {W, H,
+// tiles_x, tile_base} and pix[v] the float offset of view v's image.
 template <int TS>
-__global__ __launch_bounds__(64) void k_raster_queue(const uint4* __restrict__ views,
-                                                     const uint64_t* __restrict__ pix, uint32_t num_views,
-                                                     uint32_t total_tiles, uint32_t splats, const uint32_t* n_dev,
-                                                     uint32_t* ticket, const uint2* __restrict__ ranges,
-                                                     const uint32_t* __restrict__ vals,
-                                                     const float4* __restrict__ conic,
-                                                     const float2* __restrict__ xy, const float4* __restrict__ rgb,
-                                                     float* __restrict__ out) {
+__global__ __launch_bounds__(256) void k_raster_queue(const uint4* __restrict__ views,
+                                                      const uint64_t* __restrict__ pix, uint32_t num_views,
+                                                      uint32_t total_tiles, uint32_t splats, const uint32_t* n_dev,
+                                                      uint32_t* ticket, const uint2* __restrict__ ranges,
+                                                      const uint32_t* __restrict__ vals,
+                                                      const float4* __restrict__ conic,
+                                                      const float2* __restrict__ xy, const float4* __restrict__ rgb,
+                                                      float* __restrict__ out) {
+    constexpr uint32_t SIDE = TS / 8, BLOCKS = SIDE * SIDE;
+    __shared__ uint32_t s_ticket[2];
     const uint32_t n_inst = *n_dev;
-    for (;;) {
-        uint32_t t = 0;
-        if ((threadIdx.x & 63) == 0) t = atomicAdd(ticket, 1u);
-        t = (uint32_t)__builtin_amdgcn_readfirstlane((int)t);
-        if (t >= total_tiles) break;
+    const uint32_t items = total_tiles * BLOCKS;
+    const uint32_t tickets = ceil_div(items, 4u);
+    for (uint32_t round = 0;; ++round) {
+        // Double-buffered slot: one barrier per ticket.
+        if (threadIdx.x == 0) s_ticket[round & 1] = atomicAdd(ticket, 1u);
+        __syncthreads();
+        const uint32_t k = s_ticket[round & 1];
+        if (k >= tickets) break;  // uniform per workgroup
+        const uint32_t it = k * 4 + (threadIdx.x >> 6);
+        if (it >= items) continue;
+        const uint32_t t = it / BLOCKS, sub = it % BLOCKS;
         // view holding global tile t: last v with tile_base <= t
         uint32_t lo = 0, hi = num_views;
         while (hi - lo > 1) {
@@ -200,8 +193,8 @@ __global__ __launch_bounds__(64) void k_raster_queue(const uint4* __restrict__ v
         const uint32_t local = t - vi.w;
         const uint32_t tx = local % vi.z, ty = local / vi.z;
         const uint2 r = ranges[t];
-        raster_tile<TS>(vi.x, vi.y, tx * TS, ty * TS, r.x, min(r.y, n_inst), splats, vals, conic, xy, rgb,
-                        out + pix[lo], nullptr, nullptr);
+        raster_block(vi.x, vi.y, tx * TS + (sub % SIDE) * 8, ty * TS + (sub / SIDE) * 8, r.x, min(r.y, n_inst), splats,
+                     vals, conic, xy, rgb, out + pix[lo], nullptr, nullptr);
     }
 }
 
@@ -237,19 +230,21 @@ int launch_raster_queue(Context* c, uint32_t tile, const uint4* views, const uin
                         const uint32_t* ranges, const uint32_t* vals, const float4* conic, const float2* xy,
                         const float4* rgb, float* out, uint32_t waves_per_cu) {
     if (total_tiles == 0) return GSR_OK;
-    const uint32_t waves = std::min<uint32_t>(total_tiles, (uint32_t)c->cu_count * (waves_per_cu ? waves_per_cu : 16));
+    const uint32_t tickets = ceil_div(total_tiles * (tile * tile / 64), 4u);
+    const uint32_t groups =
+        std::max<uint32_t>(1, std::min<uint32_t>(tickets, (uint32_t)c->cu_count * (waves_per_cu ? waves_per_cu : 16) / 4));
     const uint2* rg = (const uint2*)ranges;
     switch (tile) {
         case 8:
-            hipLaunchKernelGGL(k_raster_queue<8>, dim3(waves), dim3(64), 0, c->stream, views, pix, num_views,
+            hipLaunchKernelGGL(k_raster_queue<8>, dim3(groups), dim3(256), 0, c->stream, views, pix, num_views,
                                total_tiles, splats, n_dev, ticket, rg, vals, conic, xy, rgb, out);
             break;
         case 16:
-            hipLaunchKernelGGL(k_raster_queue<16>, dim3(waves), dim3(64), 0, c->stream, views, pix, num_views,
+            hipLaunchKernelGGL(k_raster_queue<16>, dim3(groups), dim3(256), 0, c->stream, views, pix, num_views,
                                total_tiles, splats, n_dev, ticket, rg, vals, conic, xy, rgb, out);
             break;
         case 32:
-            hipLaunchKernelGGL(k_raster_queue<32>, dim3(waves), dim3(64), 0, c->stream, views, pix, num_views,
+            hipLaunchKernelGGL(k_raster_queue<32>, dim3(groups), dim3(256), 0, c->stream, views, pix, num_views,
                                total_tiles, splats, n_dev, ticket, rg, vals, conic, xy, rgb, out);
             break;
         default:
@@ -266,18 +261,19 @@ int launch_raster(Context* c, uint32_t W, uint32_t H, uint32_t tile, uint32_t sp
     const uint32_t tiles = tx * ty;
     if (tiles == 0) return GSR_OK;
     const uint2* rg = (const uint2*)ranges;
+    const uint32_t blocks = ceil_div(tiles * (tile * tile / 64), 4u);
     switch (tile) {
         case 8:
-            hipLaunchKernelGGL(k_raster<8>, dim3(tiles), dim3(64), 0, c->stream, W, H, tx, splats, n_dev, n_host, rg, vals, conic, xy,
-                               rgb, out, tfinal, last);
+            hipLaunchKernelGGL(k_raster<8>, dim3(blocks), dim3(256), 0, c->stream, W, H, tx, tiles, splats, n_dev,
+                               n_host, rg, vals, conic, xy, rgb, out, tfinal, last);
             break;
         case 16:
-            hipLaunchKernelGGL(k_raster<16>, dim3(tiles), dim3(64), 0, c->stream, W, H, tx, splats, n_dev, n_host, rg, vals, conic, xy,
-                               rgb, out, tfinal, last);
+            hipLaunchKernelGGL(k_raster<16>, dim3(blocks), dim3(256), 0, c->stream, W, H, tx, tiles, splats, n_dev,
+                               n_host, rg, vals, conic, xy, rgb, out, tfinal, last);
             break;
         case 32:
-            hipLaunchKernelGGL(k_raster<32>, dim3(tiles), dim3(64), 0, c->stream, W, H, tx, splats, n_dev, n_host, rg, vals, conic, xy,
-                               rgb, out, tfinal, last);
+            hipLaunchKernelGGL(k_raster<32>, dim3(blocks), dim3(256), 0, c->stream, W, H, tx, tiles, splats, n_dev,
+                               n_host, rg, vals, conic, xy, rgb, out, tfinal, last);
             break;
         default:
             return fail(c, GSR_ERR_ARGUMENT, "tile size %u not in {8, 16, 32}", tile);
