@@ -25,10 +25,6 @@
 namespace gsr {
 namespace {
 
-__device__ __forceinline__ float bcastf(float v, int lane) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
-}
-
 // Full-wave sum; lane 63 holds the total (same DPP pattern as the scans).
 __device__ __forceinline__ float wave_sum_f32(float v) {
     v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xF, 0xF, false));
@@ -98,8 +94,18 @@ __device__ __forceinline__ float fold8(const float (&v)[8]) {
     return z;
 }
 
+// One gathered splat as staged in LDS (colour w carries the splat index).
+struct SplatB {
+    float4 co;  // conic A, B, C and opacity
+    float4 cl;  // colour, splat index bits
+    float2 p;   // pixel position
+    float2 pad;
+};
+
 // One 8x8 block of one tile per wave64 (one pixel per lane); workgroups of 4
-// waves, workgroup ids remapped per XCD as in the forward.
+// waves, workgroup ids remapped per XCD as in the forward.  Each wave stages
+// its batch of 64 splats in its own LDS slots and reads them back with
+// same-address loads.
 template <int TS>
 __global__ __launch_bounds__(256) void k_raster_backward(
     uint32_t W, uint32_t H, uint32_t tiles_x, uint32_t num_tiles, uint32_t splats, const uint32_t* n_dev,
@@ -107,6 +113,7 @@ __global__ __launch_bounds__(256) void k_raster_backward(
     const float2* __restrict__ xy, const float4* __restrict__ rgb, const float* __restrict__ tfinal,
     const uint32_t* __restrict__ last, const float* __restrict__ dLdC, float* __restrict__ grad2d) {
     constexpr uint32_t SIDE = TS / 8, BLOCKS = SIDE * SIDE;
+    __shared__ SplatB s_spl[4][64];
     const uint32_t block = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t wv = block * 4 + (threadIdx.x >> 6);
     const uint32_t tile = wv / BLOCKS, sub = wv % BLOCKS;
@@ -137,6 +144,7 @@ __global__ __launch_bounds__(256) void k_raster_backward(
     const uint32_t tag = fold_tags();
     const float rxa = (float)x0, rxb = (float)min(x0 + 7, W - 1);
     const float rya = (float)y0, ryb = (float)min(y0 + 7, H - 1);
+    SplatB* sw = s_spl[threadIdx.x >> 6];
     for (int64_t b = (int64_t)start + (int64_t)((stop - start - 1) / 64) * 64; b >= (int64_t)start; b -= 64) {
         const uint32_t idx = (uint32_t)b + lane;
         float4 co = make_float4(0.f, 0.f, 0.f, 0.f), cl = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -153,55 +161,59 @@ __global__ __launch_bounds__(256) void k_raster_backward(
                 live = reaches_rect(p.x, p.y, co.x, co.y, co.z, reach_qmax(co.w), rxa, rxb, rya, ryb);
             }
         }
+        // Stage the batch in this wave's LDS slots (as in the forward).
+        __builtin_amdgcn_wave_barrier();
+        sw[lane].co = co;
+        sw[lane].cl = make_float4(cl.x, cl.y, cl.z, __uint_as_float(gid));
+        sw[lane].p = p;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         // live splats back to front (highest lane first)
         for (uint64_t lm = ballot(live); lm != 0;) {
             const int j = 63 - (int)__builtin_clzll(lm);
             lm &= ~(1ull << j);
             const uint32_t pos = (uint32_t)b - start + (uint32_t)j + 1;
-            const float gx = bcastf(p.x, j), gy = bcastf(p.y, j);
-            const float A = bcastf(co.x, j), B = bcastf(co.y, j), C = bcastf(co.z, j), op = bcastf(co.w, j);
-            const float cr = bcastf(cl.x, j), cg = bcastf(cl.y, j), cb = bcastf(cl.z, j);
+            const SplatB s = sw[j];
+            const float A = s.co.x, B = s.co.y, C = s.co.z, op = s.co.w;
+            const float cr = s.cl.x, cg = s.cl.y, cb = s.cl.z;
+            const float dx = s.p.x - fx, dy = s.p.y - fy;
+            const float q = __builtin_fmaf(A * dx, dx, (C * dy) * dy);
+            const float power = __builtin_fmaf(-0.5f, q, -((B * dx) * dy));
+            const float e = exp_core(power);
+            const float oe = op * e;
+            const float alpha = fminf(kAlphaClamp, oe);
+            // blended in the forward: the forward's skip tests, as predicates
+            const bool hit = (pos <= lst) & !(power > 0.0f) & !(power < -5.55f) & !(alpha < kAlphaMin);
             float a[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};  // x, y, A, B, C, opacity, r, g
             float a_b = 0.f;
-            bool any = false;
-            if (pos <= lst) {
-                const float dx = gx - fx, dy = gy - fy;
-                const float q = __builtin_fmaf(A * dx, dx, (C * dy) * dy);
-                const float power = __builtin_fmaf(-0.5f, q, -((B * dx) * dy));
-                if (!(power > 0.0f || power < -5.55f)) {
-                    const float e = exp_core(power);
-                    const float oe = op * e;
-                    const float alpha = fminf(kAlphaClamp, oe);
-                    if (alpha >= kAlphaMin) {
-                        // blended in the forward: undo it (v_rcp_f32: 1 ulp, no IEEE division sequence)
-                        const float Tb = T * __builtin_amdgcn_rcpf(1.0f - alpha);
-                        T = Tb;
-                        const float gal = ((cr - Br) * gr + (cg - Bg) * gg + (cb - Bb) * gb) * Tb;
-                        const float w = alpha * Tb;
-                        a[6] = w * gr;
-                        a[7] = w * gg;
-                        a_b = w * gb;
-                        if (oe <= kAlphaClamp) {
-                            const float gpow = alpha * gal;
-                            a[5] = e * gal;
-                            a[2] = gpow * (-0.5f * dx * dx);
-                            a[3] = gpow * (-dx * dy);
-                            a[4] = gpow * (-0.5f * dy * dy);
-                            a[0] = gpow * (-(A * dx + B * dy));
-                            a[1] = gpow * (-(C * dy + B * dx));
-                        }
-                        const float om = 1.0f - alpha;
-                        Br = alpha * cr + om * Br;
-                        Bg = alpha * cg + om * Bg;
-                        Bb = alpha * cb + om * Bb;
-                        any = true;
-                    }
+            if (hit) {
+                // undo the blend (v_rcp_f32: 1 ulp, no IEEE division sequence)
+                const float Tb = T * __builtin_amdgcn_rcpf(1.0f - alpha);
+                T = Tb;
+                const float gal = ((cr - Br) * gr + (cg - Bg) * gg + (cb - Bb) * gb) * Tb;
+                const float w = alpha * Tb;
+                a[6] = w * gr;
+                a[7] = w * gg;
+                a_b = w * gb;
+                if (oe <= kAlphaClamp) {
+                    const float gpow = alpha * gal;
+                    a[5] = e * gal;
+                    a[2] = gpow * (-0.5f * dx * dx);
+                    a[3] = gpow * (-dx * dy);
+                    a[4] = gpow * (-0.5f * dy * dy);
+                    a[0] = gpow * (-(A * dx + B * dy));
+                    a[1] = gpow * (-(C * dy + B * dx));
                 }
+                const float om = 1.0f - alpha;
+                Br = alpha * cr + om * Br;
+                Bg = alpha * cg + om * Bg;
+                Bb = alpha * cb + om * Bb;
             }
-            if (ballot(any) == 0) continue;
+            if (ballot(hit) == 0) continue;
             const float s8 = fold8(a);
             const float sb = wave_sum_f32(a_b);
-            const uint32_t g = (uint32_t)__builtin_amdgcn_readlane((int)gid, j);
+            const uint32_t g = __float_as_uint(s.cl.w);
             if (g < splats) {
                 float* dst = grad2d + (size_t)g * 12;
                 if ((lane & 7) == 0) unsafeAtomicAdd(dst + tag, s8);

@@ -5,16 +5,17 @@
 // for 8, 16, 32-pixel tiles), one pixel per lane; every wave of a tile walks
 // the tile's whole instance range, so each pixel sees the reference's
 // per-tile order.  Each batch of 64 instances is gathered one per lane
-// (value -> conic/opacity, pixel, colour) and broadcast to the wave with
-// v_readlane (no LDS, no barriers); the batch loop stops once every pixel of
-// the block is saturated, as the reference's per-batch vote does per tile.
+// (value -> conic/opacity, pixel, colour), staged in the wave's own LDS slots
+// and read back per splat with same-address (broadcast) loads, which keeps
+// the broadcast off the VALU; the batch loop stops once every pixel of the
+// block is saturated, as the reference's per-batch vote does per tile.
 // Per pixel the FP32 steps are the oracle's:
 //   power = fma(-0.5, fma(A dx, dx, (C dy) dy), -(B dx) dy); skip if > 0;
 //   alpha = min(0.99, opacity * exp(power)); skip if < 1/255;
 //   T' = T (1 - alpha); stop before blending when T' < 1e-4;
 //   rgb += colour * (alpha T); T = T'.
-// A power below -5.55 is skipped before the exp: exp(-5.55) < 1/255 and the
-// opacity is at most 1, so the reference's alpha test would reject it too.
+// A power below -5.55 is skipped too: exp(-5.55) < 1/255 and the opacity is
+// at most 1, so the reference's alpha test would reject it as well.
 // Per batch each lane also tests its splat against the 8x8 block (cull.h):
 // splats that cannot reach any pixel of the block are skipped without
 // changing any pixel's result.
@@ -52,18 +53,25 @@ __global__ __launch_bounds__(256) void k_fill_u32(uint32_t* __restrict__ p, uint
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) p[i] = v;
 }
 
-__device__ __forceinline__ float bcast(float v, int lane) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
-}
+// One gathered splat as staged in LDS: 48 bytes, read back with three
+// same-address loads (the LDS broadcasts them to the wave).
+struct SplatL {
+    float4 co;  // conic A, B, C and opacity
+    float4 cl;  // colour (w unused)
+    float2 p;   // pixel position
+    float2 pad;
+};
 
 // One 8x8 block of one tile of one view, run by one wave64, one pixel per
 // lane: lane l owns pixel (x0 + l % 8, y0 + l / 8).  start/end delimit the
 // tile's instances; out/tfinal/last address the view's image ([3][H][W]).
+// The caller's workgroup has 4 waves (wave w stages its batch in s_spl[w]).
 __device__ __forceinline__ void raster_block(uint32_t W, uint32_t H, uint32_t x0, uint32_t y0, uint32_t start,
                                             uint32_t end, uint32_t splats, const uint32_t* __restrict__ vals,
                                             const float4* __restrict__ conic, const float2* __restrict__ xy,
                                             const float4* __restrict__ rgb, float* __restrict__ out,
                                             float* __restrict__ tfinal, uint32_t* __restrict__ last) {
+    __shared__ SplatL s_spl[4][64];
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t x = x0 + lane % 8, y = y0 + lane / 8;
     const bool inside = x < W && y < H;
@@ -88,30 +96,39 @@ __device__ __forceinline__ void raster_block(uint32_t W, uint32_t H, uint32_t x0
                 live = reaches_rect(p.x, p.y, co.x, co.y, co.z, reach_qmax(co.w), rxa, rxb, rya, ryb);
             }
         }
-        // Only splats that can reach the block, in range order.
+        // Stage the batch in this wave's LDS slots.  Only this wave touches
+        // them: a wave-level barrier orders the previous batch's reads before
+        // these writes and these writes before the reads below.
+        SplatL* sw = s_spl[threadIdx.x >> 6];
+        __builtin_amdgcn_wave_barrier();
+        sw[lane].co = co;
+        sw[lane].cl = cl;
+        sw[lane].p = p;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // Only splats that can reach the block, in range order.  The pixel
+        // tests are evaluated as predicates (no per-test branches); they are
+        // the skip/stop conditions above, in the same order of effect.
         for (uint64_t lm = ballot(live); lm != 0; lm &= lm - 1) {
             const int j = (int)__builtin_ctzll(lm);
-            const float gx = bcast(p.x, j), gy = bcast(p.y, j);
-            const float A = bcast(co.x, j), B = bcast(co.y, j), C = bcast(co.z, j), op = bcast(co.w, j);
-            const float cr = bcast(cl.x, j), cg = bcast(cl.y, j), cb = bcast(cl.z, j);
-            if (done) continue;
-            const float dx = gx - fx, dy = gy - fy;
-            const float q = __builtin_fmaf(A * dx, dx, (C * dy) * dy);
-            const float power = __builtin_fmaf(-0.5f, q, -((B * dx) * dy));
-            if (power > 0.0f || power < -5.55f) continue;
-            const float alpha = fminf(kAlphaClamp, op * exp_core(power));
-            if (alpha < kAlphaMin) continue;
+            const SplatL s = sw[j];
+            const float dx = s.p.x - fx, dy = s.p.y - fy;
+            const float q = __builtin_fmaf(s.co.x * dx, dx, (s.co.z * dy) * dy);
+            const float power = __builtin_fmaf(-0.5f, q, -((s.co.y * dx) * dy));
+            const float alpha = fminf(kAlphaClamp, s.co.w * exp_core(power));
             const float tn = T * (1.0f - alpha);
-            if (tn < kTransmitMin) {
-                done = true;
-                continue;
+            const bool ok = !done & !(power > 0.0f) & !(power < -5.55f) & !(alpha < kAlphaMin);
+            const bool low = tn < kTransmitMin;
+            done |= ok & low;
+            if (ok & !low) {
+                const float w = alpha * T;
+                c0 = __builtin_fmaf(s.cl.x, w, c0);
+                c1 = __builtin_fmaf(s.cl.y, w, c1);
+                c2 = __builtin_fmaf(s.cl.z, w, c2);
+                T = tn;
+                lst = b - start + (uint32_t)j + 1;
             }
-            const float w = alpha * T;
-            c0 = __builtin_fmaf(cr, w, c0);
-            c1 = __builtin_fmaf(cg, w, c1);
-            c2 = __builtin_fmaf(cb, w, c2);
-            T = tn;
-            lst = b - start + (uint32_t)j + 1;
         }
     }
     if (inside) {
