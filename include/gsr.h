@@ -66,7 +66,7 @@ typedef struct gsr_options {
     uint32_t timing;           /* 1 = record hipEvents around every stage (gsr_get_timings)      */
     uint32_t save_state;       /* 1 = keep per-pixel T_final / last contributor for backward     */
     uint32_t raster_waves_per_cu; /* batches: persistent raster waves per CU; 0 = 16              */
-    uint32_t sort_blocks_per_cu;  /* onesweep blocks per CU; 0 = 3                                 */
+    uint32_t sort_blocks_per_cu;  /* onesweep blocks per CU: 1, 2 or 4 (1024-, 512-, 256-thread); 0 = 1 */
 } gsr_options;
 
 /* ---- timings --------------------------------------------------------- */

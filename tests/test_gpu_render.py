@@ -87,6 +87,29 @@ def test_render_1m_1080p_matches_oracle(torch_cuda, ctx, orc):
     ctx.scene_free("big")
 
 
+@pytest.mark.parametrize("key_bytes", [lib.KEY_32, lib.KEY_64])
+def test_sort_geometries_give_the_same_frame(torch_cuda, ctx, orc, key_bytes):
+    """Onesweep with 1, 2 and 4 blocks per CU (1024-, 512-, 256-thread blocks,
+    different tile sizes and look-back chains) must give the same frame."""
+    sc = scenes.clustered_scene(300_000, seed=12)
+    cam = scenes.Camera((0.5, -0.3, -7.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 55.0, 1280, 720)
+    u = cam.uniform(sc.count)
+    ctx.scene_upload("geo", sc.packed(), sc.count)
+    ctx.scene_select("geo")
+    imgs = {}
+    for blocks in (1, 2, 4):
+        _say(f"sort geometry: {blocks} blocks/CU, key bytes {key_bytes}")
+        imgs[blocks] = ctx.render(u, lib.options(key_bytes=key_bytes, sort_blocks_per_cu=blocks))
+        assert np.array_equal(imgs[blocks], imgs[1]), blocks
+    ref, _, _, Iref, _ = orc.render(sc.packed(), sc.count, u, use_radix=True)
+    assert ctx.instance_count() == Iref
+    assert np.array_equal(imgs[1], ref)
+    with pytest.raises(lib.GsrError) as e:
+        ctx.render(u, lib.options(sort_blocks_per_cu=3))
+    assert e.value.code == lib.GSR_ERR_ARGUMENT
+    ctx.scene_free("geo")
+
+
 def test_capacity_overflow_is_reported(torch_cuda, ctx):
     sc, cam = scenes.workload_small()
     ctx.scene_upload("small", sc.packed(), sc.count)

@@ -39,7 +39,9 @@ static int plan_batch(Context* c, uint32_t count, const gsr_camera* cams, uint32
     if (p->tile != 8 && p->tile != 16 && p->tile != 32)
         return fail(c, GSR_ERR_ARGUMENT, "tile size %u not in {8, 16, 32}", p->tile);
     p->raster_waves = (opts && opts->raster_waves_per_cu) ? opts->raster_waves_per_cu : 16;
-    p->sort_blocks = (opts && opts->sort_blocks_per_cu) ? opts->sort_blocks_per_cu : 3;
+    p->sort_blocks = (opts && opts->sort_blocks_per_cu) ? opts->sort_blocks_per_cu : 1;
+    if (p->sort_blocks != 1 && p->sort_blocks != 2 && p->sort_blocks != 4)
+        return fail(c, GSR_ERR_ARGUMENT, "sort blocks per CU %u not in {1, 2, 4}", p->sort_blocks);
     p->info.resize(views);
     p->pix.resize(views);
     uint64_t tiles = 0, floats = 0;
@@ -140,7 +142,6 @@ static int enqueue_batch(Context* c, const float* params, uint32_t count, const 
     GSR_HIP(c, hipMemcpyAsync(tables, c->staging, tbytes, hipMemcpyHostToDevice, c->stream));
     int rc;
     if ((rc = launch_fill_u32(c, (uint32_t*)counters, 0u, sizeof(FrameCounters) / 4))) return rc;
-    c->sort_blocks_per_cu = p.sort_blocks;
     mark(c, "preprocess");
     if ((rc = launch_preprocess_batch(c, params, count, cams_dev, p.views, p.tile, tiles, meta, conic, xy, rgb)))
         return rc;
@@ -151,8 +152,8 @@ static int enqueue_batch(Context* c, const float* params, uint32_t count, const 
         return rc;
     mark(c, "sort");
     int idx = 0;
-    if ((rc = launch_sort(c, p.key_bytes, k0, v0, k1, v1, &counters->n, 0, p.capacity, 0, p.end_bit, counters,
-                          &idx)))
+    if ((rc = launch_sort(c, p.key_bytes, k0, v0, k1, v1, &counters->n, 0, p.capacity, 0, p.end_bit,
+                          p.sort_blocks, counters, &idx)))
         return rc;
     mark(c, "tile ranges");
     void* ks = idx ? k1 : k0;
@@ -295,7 +296,7 @@ int gsr_autotune_batch(gsr_context* ctx, const float* params_dev, uint32_t count
     hipSetDevice(c->device);
     const uint32_t tiles[3] = {8, 16, 32};
     const uint32_t waves[3] = {8, 16, 32};
-    const uint32_t blocks[2] = {2, 3};
+    const uint32_t blocks[3] = {1, 2, 4};
     hipEvent_t e0, e1;
     GSR_HIP(c, hipEventCreate(&e0));
     GSR_HIP(c, hipEventCreate(&e1));
@@ -306,7 +307,7 @@ int gsr_autotune_batch(gsr_context* ctx, const float* params_dev, uint32_t count
     int rc = GSR_OK;
     for (uint32_t ti = 0; ti < 3 && rc == GSR_OK; ++ti)
         for (uint32_t wi = 0; wi < 3 && rc == GSR_OK; ++wi)
-            for (uint32_t bi = 0; bi < 2 && rc == GSR_OK; ++bi) {
+            for (uint32_t bi = 0; bi < 3 && rc == GSR_OK; ++bi) {
                 gsr_options o = *inout;
                 o.tile_size = tiles[ti];
                 o.raster_waves_per_cu = waves[wi];

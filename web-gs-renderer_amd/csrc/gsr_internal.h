@@ -67,9 +67,6 @@ static_assert(sizeof(CamK) == 176, "camera block must be 176 bytes");
 static_assert(sizeof(gsr_camera) == 176, "gsr_camera must be 176 bytes");
 
 // ---- sort tuning ---------------------------------------------------------
-constexpr int kSortThreads = 256;
-constexpr int kSortItems = 16;
-constexpr int kSortTile = kSortThreads * kSortItems;  // keys per onesweep tile
 constexpr int kRadixBits = 8;
 constexpr int kRadix = 1 << kRadixBits;
 constexpr int kMaxPasses = 8;
@@ -134,8 +131,6 @@ struct Context {
     uint64_t state_capacity = 0;
     int state_sorted_index = 0;
     uint32_t last_frame_w = 0, last_frame_h = 0;  // size of the frame in "image"
-    // launch shape knobs (set per call from gsr_options)
-    uint32_t sort_blocks_per_cu = 3;
     // prefix of the scan/sort scratch buffer names ("" plain frames, "b_" batches)
     const char* scratch_prefix = "";
     // batches
@@ -183,10 +178,11 @@ int launch_scan(Context* c, const uint32_t* in, uint32_t n, uint32_t* out, Frame
                 uint64_t capacity);
 int launch_emit(Context* c, const uint32_t* offsets, const uint4* meta, uint32_t n, uint32_t tiles_x,
                 uint64_t capacity, uint32_t key_bytes, void* keys, uint32_t* vals);
-// Sort of n (device counter n_dev if non-null, else n_host) pairs; returns result index.
+// Sort of n (device counter n_dev if non-null, else n_host) pairs with
+// onesweep blocks_per_cu in {1, 2, 4}; returns the result index.
 int launch_sort(Context* c, uint32_t key_bytes, void* k0, uint32_t* v0, void* k1, uint32_t* v1,
                 const uint32_t* n_dev, uint64_t n_host, uint64_t capacity, uint32_t begin_bit,
-                uint32_t end_bit, FrameCounters* counters, int* result_index);
+                uint32_t end_bit, uint32_t blocks_per_cu, FrameCounters* counters, int* result_index);
 int launch_ranges(Context* c, const void* keys, uint32_t key_bytes, const uint32_t* n_dev,
                   uint64_t n_host, uint32_t num_tiles, uint32_t* ranges, bool clear);
 // splats bounds the per-splat arrays, the instance count (n_dev, else n_host) the ranges.

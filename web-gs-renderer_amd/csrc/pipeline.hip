@@ -47,7 +47,9 @@ int plan_frame(Context* c, uint32_t count, const gsr_camera* cam, const gsr_opti
         cap = kMaxSortKeys;
     }
     p->capacity = cap;
-    p->sort_blocks = (opts && opts->sort_blocks_per_cu) ? opts->sort_blocks_per_cu : 3;
+    p->sort_blocks = (opts && opts->sort_blocks_per_cu) ? opts->sort_blocks_per_cu : 1;
+    if (p->sort_blocks != 1 && p->sort_blocks != 2 && p->sort_blocks != 4)
+        return fail(c, GSR_ERR_ARGUMENT, "sort blocks per CU %u not in {1, 2, 4}", p->sort_blocks);
     return GSR_OK;
 }
 
@@ -56,7 +58,6 @@ int plan_frame(Context* c, uint32_t count, const gsr_camera* cam, const gsr_opti
 int enqueue_forward(Context* c, const float* params, uint32_t count, const gsr_camera* cam, const FramePlan& p,
                     bool save_state, float* image_override) {
     const size_t N = count;
-    c->sort_blocks_per_cu = p.sort_blocks;
     uint32_t* tiles = (uint32_t*)buffer(c, "tiles", N * 4);
     uint32_t* offsets = (uint32_t*)buffer(c, "offsets", (N + 1) * 4);
     uint4* meta = (uint4*)buffer(c, "meta", N * 16);
@@ -89,8 +90,8 @@ int enqueue_forward(Context* c, const float* params, uint32_t count, const gsr_c
     if ((rc = launch_emit(c, offsets, meta, count, p.tiles_x, p.capacity, p.key_bytes, k0, v0))) return rc;
     mark(c, "sort");
     int idx = 0;
-    if ((rc = launch_sort(c, p.key_bytes, k0, v0, k1, v1, &counters->n, 0, p.capacity, 0, p.end_bit, counters,
-                          &idx)))
+    if ((rc = launch_sort(c, p.key_bytes, k0, v0, k1, v1, &counters->n, 0, p.capacity, 0, p.end_bit,
+                          p.sort_blocks, counters, &idx)))
         return rc;
     mark(c, "tile ranges");
     void* ks = idx ? k1 : k0;
@@ -261,7 +262,7 @@ int gsr_sort_pairs(gsr_context* ctx, uint32_t key_bytes, void* keys0_dev, uint32
     if (!counters) return GSR_ERR_ALLOC;
     int rc = launch_fill_u32(c, (uint32_t*)counters, 0u, sizeof(FrameCounters) / 4);
     if (!rc) rc = launch_sort(c, key_bytes, keys0_dev, values0_dev, keys1_dev, values1_dev, nullptr, n,
-                              n ? n : 1, begin_bit, end_bit, counters, result_index);
+                              n ? n : 1, begin_bit, end_bit, 1, counters, result_index);
     if (rc) return rc;
     GSR_HIP(c, hipMemcpyAsync(c->host_counters, counters, sizeof(FrameCounters), hipMemcpyDeviceToHost, c->stream));
     GSR_HIP(c, hipStreamSynchronize(c->stream));
