@@ -141,12 +141,18 @@ static int enqueue_batch(Context* c, const float* params, uint32_t count, const 
     const uint64_t* pix_dev = (const uint64_t*)(tables + p.views * (sizeof(CamK) + sizeof(uint4)));
     GSR_HIP(c, hipMemcpyAsync(tables, c->staging, tbytes, hipMemcpyHostToDevice, c->stream));
     int rc;
-    if ((rc = launch_fill_u32(c, (uint32_t*)counters, 0u, sizeof(FrameCounters) / 4))) return rc;
+    uint32_t* scan_status = scan_status_buffer(c, (uint32_t)NV);
+    if (!scan_status) return GSR_ERR_ALLOC;
+    // one launch clears the counters, the scan's look-back words and the tile ranges
+    const FillRegion clear[3] = {{(uint32_t*)counters, sizeof(FrameCounters) / 4},
+                                 {scan_status, scan_status_words((uint32_t)NV)},
+                                 {ranges, (size_t)p.total_tiles * 2}};
+    if ((rc = launch_zero_regions(c, clear, 3))) return rc;
     mark(c, "preprocess");
     if ((rc = launch_preprocess_batch(c, params, count, cams_dev, p.views, p.tile, tiles, meta, conic, xy, rgb)))
         return rc;
     mark(c, "offset scan");
-    if ((rc = launch_scan(c, tiles, (uint32_t)NV, offsets, counters, p.capacity))) return rc;
+    if ((rc = launch_scan(c, tiles, (uint32_t)NV, offsets, counters, p.capacity, true))) return rc;
     mark(c, "emit");
     if ((rc = launch_emit_batch(c, offsets, meta, count, p.views, info_dev, p.capacity, p.key_bytes, k0, v0)))
         return rc;
@@ -158,7 +164,7 @@ static int enqueue_batch(Context* c, const float* params, uint32_t count, const 
     mark(c, "tile ranges");
     void* ks = idx ? k1 : k0;
     uint32_t* vs = idx ? v1 : v0;
-    if ((rc = launch_ranges(c, ks, p.key_bytes, &counters->n, 0, p.total_tiles, ranges, true))) return rc;
+    if ((rc = launch_ranges(c, ks, p.key_bytes, &counters->n, 0, p.total_tiles, ranges, false))) return rc;
     mark(c, "raster");
     if ((rc = launch_raster_queue(c, p.tile, info_dev, pix_dev, p.views, p.total_tiles, (uint32_t)NV, &counters->n,
                                   &counters->raster_ticket, ranges, vs, conic, xy, rgb, out, p.raster_waves)))

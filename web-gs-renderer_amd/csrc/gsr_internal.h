@@ -174,8 +174,13 @@ int launch_preprocess_batch(Context* c, const float* params, uint32_t n, const C
                             uint32_t tile, uint32_t* tiles, uint4* meta, float4* conic, float2* xy, float4* rgb);
 int launch_emit_batch(Context* c, const uint32_t* offsets, const uint4* meta, uint32_t n, uint32_t num_views,
                       const uint4* views, uint64_t capacity, uint32_t key_bytes, void* keys, uint32_t* vals);
+// Scan of n counts; its look-back status buffer (scan_status_buffer, of
+// scan_status_words(n) words) must be zero, or is cleared first when
+// status_cleared is false.
 int launch_scan(Context* c, const uint32_t* in, uint32_t n, uint32_t* out, FrameCounters* counters,
-                uint64_t capacity);
+                uint64_t capacity, bool status_cleared = false);
+size_t scan_status_words(uint32_t n);
+uint32_t* scan_status_buffer(Context* c, uint32_t n);
 int launch_emit(Context* c, const uint32_t* offsets, const uint4* meta, uint32_t n, uint32_t tiles_x,
                 uint64_t capacity, uint32_t key_bytes, void* keys, uint32_t* vals);
 // Sort of n (device counter n_dev if non-null, else n_host) pairs with
@@ -190,6 +195,12 @@ int launch_raster(Context* c, uint32_t W, uint32_t H, uint32_t tile, uint32_t sp
                   uint32_t n_host, const uint32_t* ranges, const uint32_t* vals, const float4* conic,
                   const float2* xy, const float4* rgb, float* out, float* tfinal, uint32_t* last);
 int launch_fill_u32(Context* c, uint32_t* p, uint32_t value, size_t count);
+// Zeroes up to 4 regions (pointer, u32 count) with one launch.
+struct FillRegion {
+    uint32_t* p;
+    size_t count;
+};
+int launch_zero_regions(Context* c, const FillRegion* regions, int count);
 // Batch raster: views[v] = {W, H, tiles_x, tile_base}, pix[v] = float offset of view v's image.
 int launch_raster_queue(Context* c, uint32_t tile, const uint4* views, const uint64_t* pix, uint32_t num_views,
                         uint32_t total_tiles, uint32_t splats, const uint32_t* n_dev, uint32_t* ticket,

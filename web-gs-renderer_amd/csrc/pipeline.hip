@@ -81,11 +81,17 @@ int enqueue_forward(Context* c, const float* params, uint32_t count, const gsr_c
     std::memcpy(&ck, cam, sizeof(ck));
     ck.count = count;
     int rc;
-    if ((rc = launch_fill_u32(c, (uint32_t*)counters, 0u, sizeof(FrameCounters) / 4))) return rc;
+    uint32_t* scan_status = scan_status_buffer(c, count);
+    if (!scan_status) return GSR_ERR_ALLOC;
+    // one launch clears the counters, the scan's look-back words and the tile ranges
+    const FillRegion clear[3] = {{(uint32_t*)counters, sizeof(FrameCounters) / 4},
+                                 {scan_status, scan_status_words(count)},
+                                 {ranges, (size_t)p.num_tiles * 2}};
+    if ((rc = launch_zero_regions(c, clear, 3))) return rc;
     mark(c, "preprocess");
     if ((rc = launch_preprocess(c, params, count, ck, p.tile, tiles, meta, conic, xy, rgb))) return rc;
     mark(c, "offset scan");
-    if ((rc = launch_scan(c, tiles, count, offsets, counters, p.capacity))) return rc;
+    if ((rc = launch_scan(c, tiles, count, offsets, counters, p.capacity, true))) return rc;
     mark(c, "emit");
     if ((rc = launch_emit(c, offsets, meta, count, p.tiles_x, p.capacity, p.key_bytes, k0, v0))) return rc;
     mark(c, "sort");
@@ -96,7 +102,7 @@ int enqueue_forward(Context* c, const float* params, uint32_t count, const gsr_c
     mark(c, "tile ranges");
     void* ks = idx ? k1 : k0;
     uint32_t* vs = idx ? v1 : v0;
-    if ((rc = launch_ranges(c, ks, p.key_bytes, &counters->n, 0, p.num_tiles, ranges, true))) return rc;
+    if ((rc = launch_ranges(c, ks, p.key_bytes, &counters->n, 0, p.num_tiles, ranges, false))) return rc;
     mark(c, "raster");
     if ((rc = launch_raster(c, p.W, p.H, p.tile, count, &counters->n, 0, ranges, vs, conic, xy, rgb, image, tfinal,
                              last)))

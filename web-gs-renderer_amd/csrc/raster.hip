@@ -26,31 +26,62 @@
 namespace gsr {
 namespace {
 
+// Blocks take 4096-key chunks in a grid-stride loop; each wave loads 16
+// rows of 64 consecutive keys (coalesced, all in flight), and a key's
+// predecessor comes from the neighbouring lane (DPP wave_shr:1), from lane 63
+// of the previous row, or, for the wave's first key, from memory.
 template <typename K>
 __global__ __launch_bounds__(256) void k_ranges(const K* __restrict__ keys, const uint32_t* n_dev,
                                                 uint64_t n_host, uint32_t num_tiles,
                                                 uint32_t* __restrict__ ranges) {
+    constexpr int ITEMS = 16;
     const uint32_t n = n_dev ? *n_dev : (uint32_t)n_host;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const uint32_t t = (uint32_t)(keys[i] >> 16);
-        if (t >= num_tiles) continue;
-        if (i == 0) {
-            ranges[2 * t] = 0;
-        } else {
-            const uint32_t p = (uint32_t)(keys[i - 1] >> 16);
-            if (t != p) {
-                if (p < num_tiles) ranges[2 * p + 1] = i;
-                ranges[2 * t] = i;
-            }
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (uint64_t chunk = blockIdx.x; chunk * 4096 < n; chunk += gridDim.x) {
+        const uint64_t wbase = chunk * 4096 + wave * (64 * ITEMS);
+        uint32_t tile[ITEMS];
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) {
+            const uint64_t p = wbase + i * 64 + lane;
+            tile[i] = p < n ? (uint32_t)(keys[p] >> 16) : 0u;
         }
-        if (i == n - 1) ranges[2 * t + 1] = n;
+        const uint32_t before = (wbase > 0 && wbase - 1 < n) ? (uint32_t)(keys[wbase - 1] >> 16) : 0u;
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) {
+            const uint64_t p = wbase + i * 64 + lane;
+            uint32_t prev = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)tile[i], 0x138, 0xF, 0xF, false);
+            if (lane == 0) prev = i == 0 ? before : (uint32_t)__builtin_amdgcn_readlane((int)tile[i - 1], 63);
+            const uint32_t t = tile[i];
+            if (p >= n || t >= num_tiles) continue;
+            if (p == 0) {
+                ranges[2 * t] = 0;
+            } else if (t != prev) {
+                if (prev < num_tiles) ranges[2 * prev + 1] = (uint32_t)p;
+                ranges[2 * t] = (uint32_t)p;
+            }
+            if (p == n - 1) ranges[2 * t + 1] = n;
+        }
     }
 }
 
 __global__ __launch_bounds__(256) void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, size_t count) {
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) p[i] = v;
+}
+
+struct ZeroRegions {
+    uint32_t* p[4];
+    size_t end[4];  // running end of region r in the concatenated index space
+};
+
+__global__ __launch_bounds__(256) void k_zero_regions(ZeroRegions z, int count) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const size_t total = z.end[count - 1];
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+        int r = 0;
+        while (i >= z.end[r]) ++r;
+        z.p[r][i - (r ? z.end[r - 1] : 0)] = 0u;
+    }
 }
 
 // One gathered splat as staged in LDS: 48 bytes, read back with three
@@ -225,13 +256,30 @@ int launch_fill_u32(Context* c, uint32_t* p, uint32_t value, size_t count) {
     return GSR_OK;
 }
 
+int launch_zero_regions(Context* c, const FillRegion* regions, int count) {
+    if (count < 1 || count > 4) return fail(c, GSR_ERR_ARGUMENT, "zero regions: %d regions", count);
+    ZeroRegions z{};
+    size_t total = 0;
+    for (int r = 0; r < count; ++r) {
+        z.p[r] = regions[r].p;
+        total += regions[r].count;
+        z.end[r] = total;
+    }
+    if (total == 0) return GSR_OK;
+    const uint32_t blocks = (uint32_t)std::min<size_t>((total + 255) / 256, 2048);
+    hipLaunchKernelGGL(k_zero_regions, dim3(blocks), dim3(256), 0, c->stream, z, count);
+    GSR_HIP(c, hipGetLastError());
+    return GSR_OK;
+}
+
 int launch_ranges(Context* c, const void* keys, uint32_t key_bytes, const uint32_t* n_dev, uint64_t n_host,
                   uint32_t num_tiles, uint32_t* ranges, bool clear) {
     if (clear) {
         int rc = launch_fill_u32(c, ranges, 0u, (size_t)num_tiles * 2);
         if (rc) return rc;
     }
-    const uint32_t blocks = (uint32_t)c->cu_count * 4;
+    uint32_t blocks = (uint32_t)c->cu_count * 4;  // device count: sized for any n
+    if (!n_dev) blocks = (uint32_t)std::min<uint64_t>(blocks, std::max<uint64_t>(1, ceil_div(n_host, (uint64_t)4096)));
     if (key_bytes == 8)
         hipLaunchKernelGGL(k_ranges<uint64_t>, dim3(blocks), dim3(256), 0, c->stream, (const uint64_t*)keys,
                            n_dev, n_host, num_tiles, ranges);

@@ -2,18 +2,19 @@
 // offsets (src/shaders/offset_scan.wgsl:22-68), plus the device-side sizing
 // of src/shaders/indirect_arg.wgsl:14-28 (n = min(I, capacity)).
 //
-// Single pass with decoupled look-back: 256-thread blocks take 4096-entry
-// tiles in ticket order; each thread sums 16 entries, waves scan with DPP,
-// and one lane per block chains the tile prefix through 64-bit status words
-// (flag in bits 62-63, running total in bits 0-61), so totals beyond 2^32 are
-// detected rather than wrapped.
+// Single pass with decoupled look-back: 1024-thread blocks take 16384-entry
+// tiles in ticket order (few tiles, so short look-back chains); each thread
+// sums 16 consecutive entries (four 16-byte loads when the arrays are 16-byte
+// aligned), waves scan with DPP, and one lane per block chains the tile
+// prefix through 64-bit status words (flag in bits 62-63, running total in
+// bits 0-61), so totals beyond 2^32 are detected rather than wrapped.
 #include "gsr_internal.h"
 #include "wave.h"
 
 namespace gsr {
 namespace {
 
-constexpr int kScanThreads = 256;
+constexpr int kScanThreads = 1024;
 constexpr int kScanItems = 16;
 constexpr int kScanTile = kScanThreads * kScanItems;
 constexpr uint64_t kFlagAgg = 1ull << 62, kFlagInc = 2ull << 62, kValMask = (1ull << 62) - 1;
@@ -21,13 +22,17 @@ constexpr int kLookBack = 16;
 
 // counts[0..n) -> out[0..n], out[n] = total.  status[0] is the tile ticket,
 // status[1 + t] the look-back word of tile t; all zeroed before the launch.
+// VEC: in and out are 16-byte aligned (vector loads and stores away from the
+// tail).
+template <bool VEC>
 __global__ __launch_bounds__(kScanThreads) void k_scan(const uint32_t* in, uint32_t n,
                                                        uint32_t* out,  // may alias in
                                                        uint64_t* __restrict__ status_base,
                                                        FrameCounters* counters, uint64_t capacity) {
+    constexpr int WAVES = kScanThreads / 64;
     uint64_t* status = status_base + 1;
     __shared__ uint32_t s_tile;
-    __shared__ uint32_t s_wave[kScanThreads / 64];
+    __shared__ uint32_t s_wave[WAVES];
     __shared__ uint64_t s_prefix;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (tid == 0) s_tile = atomicAdd((uint32_t*)status_base, 1u);
@@ -37,19 +42,32 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(const uint32_t* in, uint3
     const uint32_t num_tiles = ceil_div(total_len, kScanTile);
     const uint64_t base = (uint64_t)tile * kScanTile + (uint64_t)tid * kScanItems;
     uint32_t v[kScanItems];
+    if (VEC && base + kScanItems <= n) {
+        const uint4* in4 = reinterpret_cast<const uint4*>(in + base);
+#pragma unroll
+        for (int q = 0; q < kScanItems / 4; ++q) {
+            const uint4 t = in4[q];
+            v[4 * q] = t.x;
+            v[4 * q + 1] = t.y;
+            v[4 * q + 2] = t.z;
+            v[4 * q + 3] = t.w;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < kScanItems; ++i) {
+            const uint64_t p = base + i;
+            v[i] = p < n ? in[p] : 0u;
+        }
+    }
     uint32_t sum = 0;
 #pragma unroll
-    for (int i = 0; i < kScanItems; ++i) {
-        const uint64_t p = base + i;
-        v[i] = p < n ? in[p] : 0u;
-        sum += v[i];
-    }
+    for (int i = 0; i < kScanItems; ++i) sum += v[i];
     const uint32_t incl = wave_inclusive_sum(sum);
     if (lane == 63) s_wave[wave] = incl;
     __syncthreads();
     uint32_t wave_prefix = 0, block_total = 0;
 #pragma unroll
-    for (int w = 0; w < kScanThreads / 64; ++w) {
+    for (int w = 0; w < WAVES; ++w) {
         const uint32_t t = s_wave[w];
         if (w < (int)wave) wave_prefix += t;
         block_total += t;
@@ -97,25 +115,47 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(const uint32_t* in, uint3
     }
     __syncthreads();
     uint64_t run = s_prefix + wave_prefix + (incl - sum);
+    uint32_t o[kScanItems];
 #pragma unroll
     for (int i = 0; i < kScanItems; ++i) {
-        const uint64_t p = base + i;
-        if (p <= n) out[p] = (uint32_t)run;
+        o[i] = (uint32_t)run;
         run += v[i];
+    }
+    if (VEC && base + kScanItems <= (uint64_t)n + 1) {
+        uint4* out4 = reinterpret_cast<uint4*>(out + base);
+#pragma unroll
+        for (int q = 0; q < kScanItems / 4; ++q) out4[q] = make_uint4(o[4 * q], o[4 * q + 1], o[4 * q + 2], o[4 * q + 3]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < kScanItems; ++i)
+            if (base + i <= n) out[base + i] = o[i];
     }
 }
 
 }  // namespace
 
+size_t scan_status_words(uint32_t n) { return ((size_t)ceil_div((uint64_t)n + 1, kScanTile) + 1) * 2; }
+
+uint32_t* scan_status_buffer(Context* c, uint32_t n) {
+    return (uint32_t*)scratch(c, "scan_status", scan_status_words(n) * 4);
+}
+
 int launch_scan(Context* c, const uint32_t* in, uint32_t n, uint32_t* out, FrameCounters* counters,
-                uint64_t capacity) {
+                uint64_t capacity, bool status_cleared) {
     const uint32_t tiles = ceil_div((uint64_t)n + 1, kScanTile);
-    uint64_t* status = (uint64_t*)scratch(c, "scan_status", (size_t)(tiles + 1) * 8);
+    uint64_t* status = (uint64_t*)scan_status_buffer(c, n);
     if (!status) return GSR_ERR_ALLOC;
-    int rc = launch_fill_u32(c, (uint32_t*)status, 0u, (size_t)(tiles + 1) * 2);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_scan, dim3(tiles), dim3(kScanThreads), 0, c->stream, in, n, out, status, counters,
-                       capacity);
+    if (!status_cleared) {
+        const int rc = launch_fill_u32(c, (uint32_t*)status, 0u, scan_status_words(n));
+        if (rc) return rc;
+    }
+    const bool vec = ((uintptr_t)in % 16 == 0) && ((uintptr_t)out % 16 == 0);
+    if (vec)
+        hipLaunchKernelGGL(k_scan<true>, dim3(tiles), dim3(kScanThreads), 0, c->stream, in, n, out, status, counters,
+                           capacity);
+    else
+        hipLaunchKernelGGL(k_scan<false>, dim3(tiles), dim3(kScanThreads), 0, c->stream, in, n, out, status,
+                           counters, capacity);
     GSR_HIP(c, hipGetLastError());
     return GSR_OK;
 }

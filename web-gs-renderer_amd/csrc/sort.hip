@@ -7,8 +7,9 @@
 //   k_sort_init      zero digit histograms, tile tickets and look-back words
 //                    (only for the tiles the device-side count needs);
 //   k_sort_hist      one read of the keys builds every pass's 256-bin digit
-//                    histogram (per-wave LDS bins; a run of equal digits in
-//                    consecutive lanes adds once; one global atomic per bin);
+//                    histogram (LDS bins; a run of equal digits in
+//                    consecutive lanes adds once; one global atomic per bin
+//                    and block, one block per CU);
 //   k_onesweep x P   persistent blocks take tiles by ticket, rank keys per
 //                    wave64 with ballot digit matching (stable), chain
 //                    per-digit tile prefixes with decoupled look-back, stage
@@ -74,19 +75,21 @@ __global__ __launch_bounds__(256) void k_sort_init(uint32_t* __restrict__ hist_t
     }
 }
 
-// Blocks take 256 x kHistItems-key tiles in a grid-stride loop; each thread
-// issues all kHistItems loads of a tile before counting, so the loop is not
-// bound by one load's latency per key.
+// 1024-thread blocks (one per CU) take 1024 x kHistItems-key tiles in a
+// grid-stride loop; each thread issues all kHistItems loads of a tile before
+// counting, so the loop is not bound by one load's latency per key.  Waves
+// w and w + 4k share LDS copy w % 4 of the bins.
 template <typename K>
-__global__ __launch_bounds__(256) void k_sort_hist(const K* __restrict__ keys, const uint32_t* n_dev,
+__global__ __launch_bounds__(1024) void k_sort_hist(const K* __restrict__ keys, const uint32_t* n_dev,
                                                    uint64_t n_host, uint32_t begin_bit, uint32_t end_bit,
                                                    uint32_t passes, uint32_t* __restrict__ hist) {
     __shared__ uint32_t s_hist[4][kMaxPasses * kRadix];
     const uint32_t n = read_n(n_dev, n_host);
-    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (uint32_t i = threadIdx.x; i < 4 * kMaxPasses * kRadix; i += blockDim.x) (&s_hist[0][0])[i] = 0;
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, copy = wave & 3;
+    for (uint32_t w = 0; w < 4; ++w)
+        for (uint32_t i = threadIdx.x; i < passes * kRadix; i += blockDim.x) s_hist[w][i] = 0;
     __syncthreads();
-    constexpr uint32_t kBlockKeys = 256 * kHistItems;
+    constexpr uint32_t kBlockKeys = 1024 * kHistItems;
     for (uint32_t t = blockIdx.x; (uint64_t)t * kBlockKeys < n; t += gridDim.x) {
         const uint32_t wbase = t * kBlockKeys + wave * (64 * kHistItems);
         K k[kHistItems];
@@ -112,7 +115,7 @@ __global__ __launch_bounds__(256) void k_sort_hist(const K* __restrict__ keys, c
                 if (head) {
                     const uint64_t after = lane == 63 ? 0ull : heads & ~((2ull << lane) - 1ull);
                     const uint32_t next = after ? (uint32_t)__builtin_ctzll(after) : valid;
-                    atomicAdd(&s_hist[wave][p * kRadix + d], next - lane);
+                    atomicAdd(&s_hist[copy][p * kRadix + d], next - lane);
                 }
             }
         }
@@ -286,9 +289,12 @@ int sort_impl(Context* c, K* k0, uint32_t* v0, K* k1, uint32_t* v1, const uint32
     const uint32_t fill_blocks = std::min<uint32_t>(std::max<uint32_t>(1, max_tiles), 1024);
     hipLaunchKernelGGL(k_sort_init, dim3(fill_blocks), dim3(256), 0, c->stream, hist, status, n_dev, n_host,
                        passes, max_tiles, (uint32_t)G::kTile);
+    // One block per CU: every block adds its bins to the global histogram with
+    // one atomic each, and those atomics serialize per bin across blocks, so
+    // fewer, larger blocks.
     const uint32_t hist_blocks = std::min<uint32_t>(
-        std::max<uint32_t>(1, ceil_div(capacity, (uint64_t)256 * kHistItems)), (uint32_t)c->cu_count * 4);
-    hipLaunchKernelGGL(k_sort_hist<K>, dim3(hist_blocks), dim3(256), 0, c->stream, k0, n_dev, n_host,
+        std::max<uint32_t>(1, ceil_div(capacity, (uint64_t)1024 * kHistItems)), (uint32_t)c->cu_count);
+    hipLaunchKernelGGL(k_sort_hist<K>, dim3(hist_blocks), dim3(1024), 0, c->stream, k0, n_dev, n_host,
                        begin_bit, end_bit, passes, hist);
     const uint32_t sweep_blocks = std::min<uint32_t>(max_tiles, (uint32_t)c->cu_count * (1024 / THREADS));
     K* kb[2] = {k0, k1};
