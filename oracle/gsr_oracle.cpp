@@ -874,10 +874,12 @@ void backward_f64(const float* packed, uint32_t n, const Cam& cam, uint32_t tile
 }
 
 // FP32 replay of the raster adjoint with the same steps as the GPU kernel
-// (T recovered by dividing by 1 - alpha, colour-behind recurrence), per-splat
+// (T recovered by dividing by 1 - alpha, colour-behind recurrence, per-splat
+// moments of dL/dpower turned into the 2-D gradient with the conic), per-splat
 // sums in double.  The kernel multiplies by a 1-ulp hardware reciprocal
-// instead of dividing; otherwise the arithmetic is the same, so the twin
-// separates FP32 term error from summation order.
+// instead of dividing and converts the moments in FP32; otherwise the
+// arithmetic is the same, so the twin separates FP32 term error from
+// summation order.
 void raster_backward_f32(uint32_t W, uint32_t H, uint32_t tile, const uint32_t* ranges, const uint32_t* vals,
                          const float* conic, const float* xy, const float* rgb, const float* tfinal,
                          const uint32_t* last, const float* dLdC, uint32_t n, double* grad2d) {
@@ -909,25 +911,39 @@ void raster_backward_f32(uint32_t W, uint32_t H, uint32_t tile, const uint32_t* 
                     const float Tb = T / (1.0f - alpha);
                     T = Tb;
                     const float cr = rgb[g * 4], cg = rgb[g * 4 + 1], cb = rgb[g * 4 + 2];
-                    const float gal = ((cr - Br) * gr + (cg - Bg) * gg + (cb - Bb) * gb) * Tb;
+                    const float gal = fmaf(cr - Br, gr, fmaf(cg - Bg, gg, (cb - Bb) * gb)) * Tb;
                     const float w = alpha * Tb;
                     double* a = &grad2d[(size_t)g * G2];
                     a[6] += w * gr; a[7] += w * gg; a[8] += w * gb;
                     if (oe <= 0.99f) {
+                        // moments of dL/dpower, turned into the 2-D gradient below
                         const float gpow = alpha * gal;
+                        const float gdx = gpow * dx, gdy = gpow * dy;
+                        a[0] += gdx;
+                        a[1] += gdy;
+                        a[2] += gdx * dx;
+                        a[3] += gdx * dy;
+                        a[4] += gdy * dy;
                         a[5] += e * gal;
-                        a[2] += gpow * (-0.5f * dx * dx);
-                        a[3] += gpow * (-dx * dy);
-                        a[4] += gpow * (-0.5f * dy * dy);
-                        a[0] += gpow * (-(A * dx + B * dy));
-                        a[1] += gpow * (-(C * dy + B * dx));
                     }
                     const float om = 1.0f - alpha;
-                    Br = alpha * cr + om * Br;
-                    Bg = alpha * cg + om * Bg;
-                    Bb = alpha * cb + om * Bb;
+                    Br = fmaf(alpha, cr, om * Br);
+                    Bg = fmaf(alpha, cg, om * Bg);
+                    Bb = fmaf(alpha, cb, om * Bb);
                 }
             }
+    }
+    // per splat: d/dx = -(A Mx + B My), d/dy = -(C My + B Mx), d/dA = -Mxx/2,
+    // d/dB = -Mxy, d/dC = -Myy/2 with the splat's conic
+    for (uint32_t g = 0; g < n; ++g) {
+        double* a = &grad2d[(size_t)g * G2];
+        const double A = conic[g * 4], B = conic[g * 4 + 1], C = conic[g * 4 + 2];
+        const double mx = a[0], my = a[1];
+        a[0] = -(A * mx + B * my);
+        a[1] = -(C * my + B * mx);
+        a[2] *= -0.5;
+        a[3] = -a[3];
+        a[4] *= -0.5;
     }
 }
 }  // namespace

@@ -11,8 +11,9 @@
 //   way.  Per splat the 9 gradient terms are summed over the wave (8 by a
 //   swap-and-add transposition, 1 by DPP) and added to the splat's 2-D
 //   gradient with float atomics, one instruction for the 8.
-// k_preprocess_backward: one thread per splat, chains the 2-D gradient
-//   (pixel position, conic, opacity, colour) back to the 59 parameters.
+// k_preprocess_backward: one thread per splat, turns the raster's sums into
+//   the 2-D gradient (pixel position, conic, opacity, colour) and chains it
+//   back to the 59 parameters.
 //
 // 2-D gradient layout per splat (12 floats): d/dx, d/dy of the pixel
 // position; d/dA, d/dB, d/dC of the conic; d/d opacity; d/d(r, g, b); pad.
@@ -24,17 +25,6 @@
 
 namespace gsr {
 namespace {
-
-// Full-wave sum; lane 63 holds the total (same DPP pattern as the scans).
-__device__ __forceinline__ float wave_sum_f32(float v) {
-    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xF, 0xF, false));
-    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x112, 0xF, 0xF, false));
-    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x114, 0xF, 0xF, false));
-    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x118, 0xF, 0xF, false));
-    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x142, 0xA, 0xF, false));
-    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x143, 0xC, 0xF, false));
-    return v;
-}
 
 __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
 #pragma unroll
@@ -52,10 +42,6 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
 // group.  Lane 8i then holds one value's wave total.  Which value lands where
 // depends only on the swap network, so the same network run once on the
 // values' indices (fold_tags) names it.
-struct Fold8 {
-    float sum;     // wave total of value tag, valid in lanes 8i
-    uint32_t tag;  // index of that value, 0..7
-};
 
 __device__ __forceinline__ float fold32(float d, float s) {
     const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(d), __float_as_uint(s), false, false);
@@ -82,16 +68,32 @@ __device__ __forceinline__ uint32_t fold_tags() {
     return (threadIdx.x & 8) ? w1 : w0;
 }
 
-__device__ __forceinline__ float fold8(const float (&v)[8]) {
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_f32_rows(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xF, false));
+}
+
+// The 8-value fold of v (s8: lane 8i holds the total of v[fold_tags() of
+// lane 8i]) and the plain DPP wave sum of vb (sb, total in lane 63), their DPP
+// steps interleaved so that each fills the other's read-after-write wait
+// states.
+__device__ __forceinline__ void fold9(const float (&v)[8], float vb, float& s8, float& sb) {
     const float u0 = fold32(v[0], v[1]), u1 = fold32(v[2], v[3]);
     const float u2 = fold32(v[4], v[5]), u3 = fold32(v[6], v[7]);
     const float w0 = fold16(u0, u1), w1 = fold16(u2, u3);
     const bool hi = (threadIdx.x & 8) != 0;
     float z = (hi ? w1 : w0) + dpp_f32<0x128>(hi ? w0 : w1);  // row_ror:8
+    float b = vb + dpp_f32<0x111>(vb);                          // row_shr:1
     z += dpp_f32<0x141>(z);                                     // row_half_mirror
+    b += dpp_f32<0x112>(b);                                     // row_shr:2
     z += dpp_f32<0xB1>(z);                                      // quad_perm [1,0,3,2]
+    b += dpp_f32<0x114>(b);                                     // row_shr:4
     z += dpp_f32<0x4E>(z);                                      // quad_perm [2,3,0,1]
-    return z;
+    b += dpp_f32<0x118>(b);                                     // row_shr:8
+    b += dpp_f32_rows<0x142, 0xA>(b);                           // row_bcast:15
+    b += dpp_f32_rows<0x143, 0xC>(b);                           // row_bcast:31
+    s8 = z;
+    sb = b;
 }
 
 // One gathered splat as staged in LDS (colour w carries the splat index).
@@ -175,7 +177,7 @@ __global__ __launch_bounds__(256) void k_raster_backward(
             lm &= ~(1ull << j);
             const uint32_t pos = (uint32_t)b - start + (uint32_t)j + 1;
             const SplatB s = sw[j];
-            const float A = s.co.x, B = s.co.y, C = s.co.z, op = s.co.w;
+            const float A = s.co.x, B = s.co.y, C = s.co.z, op = s.co.w;  // A..C: power only
             const float cr = s.cl.x, cg = s.cl.y, cb = s.cl.z;
             const float dx = s.p.x - fx, dy = s.p.y - fy;
             const float q = __builtin_fmaf(A * dx, dx, (C * dy) * dy);
@@ -185,34 +187,31 @@ __global__ __launch_bounds__(256) void k_raster_backward(
             const float alpha = fminf(kAlphaClamp, oe);
             // blended in the forward: the forward's skip tests, as predicates
             const bool hit = (pos <= lst) & !(power > 0.0f) & !(power < -5.55f) & !(alpha < kAlphaMin);
-            float a[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};  // x, y, A, B, C, opacity, r, g
-            float a_b = 0.f;
-            if (hit) {
-                // undo the blend (v_rcp_f32: 1 ulp, no IEEE division sequence)
-                const float Tb = T * __builtin_amdgcn_rcpf(1.0f - alpha);
-                T = Tb;
-                const float gal = ((cr - Br) * gr + (cg - Bg) * gg + (cb - Bb) * gb) * Tb;
-                const float w = alpha * Tb;
-                a[6] = w * gr;
-                a[7] = w * gg;
-                a_b = w * gb;
-                if (oe <= kAlphaClamp) {
-                    const float gpow = alpha * gal;
-                    a[5] = e * gal;
-                    a[2] = gpow * (-0.5f * dx * dx);
-                    a[3] = gpow * (-dx * dy);
-                    a[4] = gpow * (-0.5f * dy * dy);
-                    a[0] = gpow * (-(A * dx + B * dy));
-                    a[1] = gpow * (-(C * dy + B * dx));
-                }
-                const float om = 1.0f - alpha;
-                Br = alpha * cr + om * Br;
-                Bg = alpha * cg + om * Bg;
-                Bb = alpha * cb + om * Bb;
-            }
+            // per-splat sums: moments of g = dL/dpower (g dx, g dy, g dx^2,
+            // g dx dy, g dy^2; the conic and position gradients follow from
+            // them per splat), dL/d opacity, dL/d(r, g); dL/db apart.
+            // Every lane evaluates the terms; lanes that did not blend this
+            // splat (or sit on the alpha clamp) select zero multipliers.
+            // alpha <= 0.99 keeps the recovered T finite in every lane.
             if (ballot(hit) == 0) continue;
-            const float s8 = fold8(a);
-            const float sb = wave_sum_f32(a_b);
+            const bool grad_ok = hit & (oe <= kAlphaClamp);
+            // undo the blend (v_rcp_f32: 1 ulp, no IEEE division sequence)
+            const float Tb = T * __builtin_amdgcn_rcpf(1.0f - alpha);
+            const float gal = __builtin_fmaf(cr - Br, gr, __builtin_fmaf(cg - Bg, gg, (cb - Bb) * gb)) * Tb;
+            const float w = hit ? alpha * Tb : 0.0f;
+            const float gpow = grad_ok ? alpha * gal : 0.0f;
+            const float gdx = gpow * dx, gdy = gpow * dy;
+            const float a[8] = {gdx, gdy, gdx * dx, gdx * dy, gdy * dy, grad_ok ? e * gal : 0.0f, w * gr, w * gg};
+            const float a_b = w * gb;
+            if (hit) {
+                T = Tb;
+                const float om = 1.0f - alpha;
+                Br = __builtin_fmaf(alpha, cr, om * Br);
+                Bg = __builtin_fmaf(alpha, cg, om * Bg);
+                Bb = __builtin_fmaf(alpha, cb, om * Bb);
+            }
+            float s8, sb;
+            fold9(a, a_b, s8, sb);
             const uint32_t g = __float_as_uint(s.cl.w);
             if (g < splats) {
                 float* dst = grad2d + (size_t)g * 12;
@@ -233,17 +232,35 @@ constexpr float SH_C3_0 = -0.5900435899266435f, SH_C3_1 = 2.890611442640554f,
                 SH_C3_4 = -0.4570457994644658f, SH_C3_5 = 1.445305721320277f,
                 SH_C3_6 = -0.5900435899266435f;
 
+// grad2d arrives with the raster's per-splat sums (moments of dL/dpower in
+// slots 0-4) and leaves in the 2-D gradient layout: d/dx = -(A Mx + B My),
+// d/dy = -(C My + B Mx), d/dA = -Mxx / 2, d/dB = -Mxy, d/dC = -Myy / 2, with
+// the splat's forward conic (A, B, C).
 __global__ __launch_bounds__(256) void k_preprocess_backward(const float* __restrict__ P, uint32_t n, CamK cam,
-                                                             const float4* __restrict__ grad2d,
+                                                             const float4* __restrict__ conic,
+                                                             float4* __restrict__ grad2d,
                                                              float* __restrict__ grad) {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
     const size_t N = n;
     const float4 G0 = grad2d[(size_t)g * 3], G1 = grad2d[(size_t)g * 3 + 1], G2 = grad2d[(size_t)g * 3 + 2];
-    const float Gv[9] = {G0.x, G0.y, G0.z, G0.w, G1.x, G1.y, G1.z, G1.w, G2.x};
     bool any = false;
+    {
+        const float m[9] = {G0.x, G0.y, G0.z, G0.w, G1.x, G1.y, G1.z, G1.w, G2.x};
 #pragma unroll
-    for (int k = 0; k < 9; ++k) any |= Gv[k] != 0.0f;
+        for (int k = 0; k < 9; ++k) any |= m[k] != 0.0f;
+    }
+    float Gv[9] = {0.f, 0.f, 0.f, 0.f, 0.f, G1.y, G1.z, G1.w, G2.x};
+    if (any) {
+        const float4 cn = conic[g];
+        Gv[0] = -__builtin_fmaf(cn.x, G0.x, cn.y * G0.y);
+        Gv[1] = -__builtin_fmaf(cn.z, G0.y, cn.y * G0.x);
+        Gv[2] = -0.5f * G0.z;
+        Gv[3] = -G0.w;
+        Gv[4] = -0.5f * G1.x;
+        grad2d[(size_t)g * 3] = make_float4(Gv[0], Gv[1], Gv[2], Gv[3]);
+        grad2d[(size_t)g * 3 + 1] = make_float4(Gv[4], G1.y, G1.z, G1.w);
+    }
     const float px = P[CH_POS * N + g], py = P[(CH_POS + 1) * N + g], pz = P[(CH_POS + 2) * N + g];
     const float* V = cam.view;
     const float* PV = cam.view_proj;
@@ -473,7 +490,7 @@ int launch_backward(Context* c, const float* params, uint32_t count, const CamK&
     GSR_HIP(c, hipGetLastError());
     mark(c, "backward preprocess");
     hipLaunchKernelGGL(k_preprocess_backward, dim3(ceil_div(count, 256)), dim3(256), 0, c->stream, params, count, cam,
-                       (const float4*)g2, grad);
+                       conic, (float4*)g2, grad);
     GSR_HIP(c, hipGetLastError());
     mark_end(c, false);
     if (grad2d_out) GSR_HIP(c, hipMemcpyAsync(grad2d_out, g2, N * 48, hipMemcpyDeviceToDevice, c->stream));
