@@ -18,12 +18,15 @@
 // FP32 arithmetic order.  The WGSL leaves the rounding of exp and the
 // contraction of a*b+c to the WebGPU implementation.  This restatement fixes
 // one order: no implicit contraction (built with -ffp-contract=off), fmaf only
-// where written, IEEE sqrt and division, and exp from exp_f32() below
-// (Cody-Waite reduction, degree-7 Taylor polynomial).  The HIP kernels follow
-// the same order, so their FP32 outputs are comparable bit for bit.  FP64
-// versions (libm exp, same formulas) measure the FP32 error.
+// where written, IEEE sqrt and division, exp from exp_f32() below
+// (Cody-Waite reduction, degree-7 Taylor polynomial) in preprocess and from
+// exp_blend_f32() (exp2 reduction, degree-5 polynomial, for powers in
+// [-5.55, 0]) in the blend.  The HIP kernels follow the same order, so their
+// FP32 outputs are comparable bit for bit.  FP64 versions (libm exp, same
+// formulas) measure the FP32 error.
 //
-// Pins (tests/test_oracle.py): exp_f32 against libm within 2 ulp; the integer
+// Pins (tests/test_oracle.py): exp_f32 against libm within 2 ulp, exp_blend_f32
+// within 2.7 ulp on [-5.55, 0]; the integer
 // chain against closed forms / brute force; preprocess special cases
 // (isotropic splat, on-axis Jacobian, SH degree 0, clamp, sigmoid(0));
 // FP32 vs FP64 agreement; raster closed forms on 1-3 splats.
@@ -77,13 +80,37 @@ inline float exp_f32(float x) {
     return ldexpf(p, (int)n);
 }
 
+// e^p for the blend's alpha, p in [-5.55, 0] (raster only; the GPU kernels
+// use the same steps): 2^(p log2 e), k = p log2 e rounded to an integer by
+// adding 1.5 * 2^23, f = fma(p, log2 e, -k), 2^f by a degree-5 polynomial
+// (coefficients fitted by iteratively reweighted least squares for relative
+// error on [-1/2, 1/2], oracle/exp_blend_fit.py), times 2^k.  Pinned within
+// 2.7 ulp of libm's exp on [-5.55, 0] (tests/test_oracle.py).
+inline float exp_blend_f32(float p) {
+    const float kLog2e = 0x1.715476p+0f, kRound = 0x1.8p+23f;
+    const float t = fmaf(p, kLog2e, kRound);
+    const float f = fmaf(p, kLog2e, -(t - kRound));
+    float q = 0x1.5bba1p-10f;
+    q = fmaf(q, f, 0x1.3cea88p-7f);
+    q = fmaf(q, f, 0x1.c6b752p-5f);
+    q = fmaf(q, f, 0x1.ebf9bcp-3f);
+    q = fmaf(q, f, 0x1.62e42ap-1f);
+    q = fmaf(q, f, 1.0f);
+    uint32_t tu, ru;
+    std::memcpy(&tu, &t, 4);
+    std::memcpy(&ru, &kRound, 4);
+    return ldexpf(q, (int32_t)(tu - ru));
+}
+
 template <typename T> struct M;
 template <> struct M<float> {
     static float ex(float x) { return exp_f32(x); }
+    static float ex_blend(float x) { return exp_blend_f32(x); }
     static float fma(float a, float b, float c) { return fmaf(a, b, c); }
 };
 template <> struct M<double> {
     static double ex(double x) { return std::exp(x); }
+    static double ex_blend(double x) { return std::exp(x); }
     static double fma(double a, double b, double c) { return std::fma(a, b, c); }
 };
 
@@ -429,14 +456,14 @@ void raster_all(uint32_t W, uint32_t H, uint32_t tile, const uint32_t* ranges, c
                     const T q = M<T>::fma(A * dx, dx, (C * dy) * dy);
                     const T power = M<T>::fma(T(-0.5), q, -((B * dx) * dy));
                     if (power > T(0)) continue;
-                    const T alpha = std::min(T(0.99f), op * M<T>::ex(power));
+                    const T alpha = std::min(T(0.99f), op * M<T>::ex_blend(power));
                     if (alpha < T(1.0f / 255.0f)) continue;
                     const T testT = Tr * (T(1) - alpha);
                     if (testT < T(0.0001f)) {
                         h = fnv(h, 0xFFFFFFFFull + i);
                         break;
                     }
-                    h = fnv(h, ((uint64_t)vals[i] << 1) | (op * M<T>::ex(power) > T(0.99f) ? 1u : 0u));
+                    h = fnv(h, ((uint64_t)vals[i] << 1) | (op * M<T>::ex_blend(power) > T(0.99f) ? 1u : 0u));
                     const T wgt = alpha * Tr;
                     c0 = M<T>::fma(rgb[g * 4], wgt, c0);
                     c1 = M<T>::fma(rgb[g * 4 + 1], wgt, c1);
@@ -765,7 +792,7 @@ void backward_f64(const float* packed, uint32_t n, const Cam& cam, uint32_t tile
                             const float q = fmaf(A * dx, dx, (C * dy) * dy);
                             const float power = fmaf(-0.5f, q, -((B * dx) * dy));
                             if (power > 0.0f) continue;
-                            const float oe = conicf[g * 4 + 3] * exp_f32(power);
+                            const float oe = conicf[g * 4 + 3] * exp_blend_f32(power);
                             const float alpha = std::min(0.99f, oe);
                             if (alpha < 1.0f / 255.0f) continue;
                             const float tn = Tr * (1.0f - alpha);
@@ -904,7 +931,7 @@ void raster_backward_f32(uint32_t W, uint32_t H, uint32_t tile, const uint32_t* 
                     const float q = fmaf(A * dx, dx, (C * dy) * dy);
                     const float power = fmaf(-0.5f, q, -((B * dx) * dy));
                     if (power > 0.0f || power < -5.55f) continue;
-                    const float e = exp_f32(power);
+                    const float e = exp_blend_f32(power);
                     const float oe = op * e;
                     const float alpha = std::min(0.99f, oe);
                     if (alpha < 1.0f / 255.0f) continue;
@@ -951,6 +978,10 @@ void raster_backward_f32(uint32_t W, uint32_t H, uint32_t tile, const uint32_t* 
 extern "C" {
 
 float orc_exp_f32(float x) { return exp_f32(x); }
+
+void orc_exp_blend_f32_array(const float* x, float* y, int64_t n) {
+    for (int64_t i = 0; i < n; ++i) y[i] = exp_blend_f32(x[i]);
+}
 
 void orc_exp_f32_array(const float* x, float* y, int64_t n) {
     for (int64_t i = 0; i < n; ++i) y[i] = exp_f32(x[i]);

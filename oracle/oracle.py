@@ -34,6 +34,7 @@ def lib():
         L.orc_exp_f32.restype = ctypes.c_float
         L.orc_exp_f32.argtypes = [ctypes.c_float]
         L.orc_exp_f32_array.argtypes = [_f32p, _f32p, ctypes.c_int64]
+        L.orc_exp_blend_f32_array.argtypes = [_f32p, _f32p, ctypes.c_int64]
         L.orc_num_threads.restype = ctypes.c_int
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_preprocess_f32.argtypes = [_f32p, ctypes.c_uint32, _f32p, ctypes.c_uint32,
@@ -73,6 +74,13 @@ def exp_f32(x):
     x = np.ascontiguousarray(x, dtype=np.float32)
     y = np.empty_like(x)
     lib().orc_exp_f32_array(x.ravel(), y.ravel(), x.size)
+    return y
+
+
+def exp_blend_f32(x):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.empty_like(x)
+    lib().orc_exp_blend_f32_array(x.ravel(), y.ravel(), x.size)
     return y
 
 

@@ -28,6 +28,26 @@ def test_exp_matches_libm(orc):
     assert np.isinf(orc.exp_f32(np.array([89.0], np.float32))[0])
 
 
+def test_exp_blend_matches_libm_on_the_blend_range(orc):
+    # every 7th float of [-5.55, 0] (about 160M values) plus the ends; the
+    # bound was measured over every float of the range
+    lo = np.float32(-5.55).view(np.uint32)
+    bits = np.arange(np.uint32(0x80000000), lo + 1, 7, dtype=np.uint32)
+    for chunk in np.array_split(bits, 16):
+        x = chunk.view(np.float32)
+        y = orc.exp_blend_f32(x).astype(np.float64)
+        ref = np.exp(x.astype(np.float64))
+        ulp = np.spacing(ref.astype(np.float32)).astype(np.float64)
+        assert (np.abs(y - ref) / ulp).max() < 2.7
+    ends = orc.exp_blend_f32(np.array([0.0, -0.0, -5.55, -np.log(255.0)], np.float32))
+    assert ends[0] == 1.0 and ends[1] == 1.0
+    assert ends[2] < np.float32(1.0 / 255.0) <= ends[3] * np.float32(1.0 + 3e-7)
+    # monotone on a dense grid (alpha thresholds depend on it)
+    x = np.linspace(-5.55, 0.0, 2_000_001, dtype=np.float32)
+    y = orc.exp_blend_f32(x)
+    assert (np.diff(y) >= 0).all()
+
+
 @pytest.mark.parametrize("n", [0, 1, 7, 4096, 100003])
 def test_scan_is_exclusive_prefix_sum(orc, n):
     rng = np.random.default_rng(n)

@@ -105,21 +105,22 @@ struct SplatB {
 };
 
 // One 8x8 block of one tile per wave64 (one pixel per lane); workgroups of 4
-// waves, workgroup ids remapped per XCD as in the forward.  Each wave stages
-// its batch of 64 splats in its own LDS slots and reads them back with
-// same-address loads.
+// waves take the tiles heaviest first (order[], as in the forward).  Each
+// wave stages its batch of 64 splats in its own LDS slots and reads them back
+// with same-address loads.
 template <int TS>
 __global__ __launch_bounds__(256) void k_raster_backward(
     uint32_t W, uint32_t H, uint32_t tiles_x, uint32_t num_tiles, uint32_t splats, const uint32_t* n_dev,
     const uint2* __restrict__ ranges, const uint32_t* __restrict__ vals, const float4* __restrict__ conic,
     const float2* __restrict__ xy, const float4* __restrict__ rgb, const float* __restrict__ tfinal,
-    const uint32_t* __restrict__ last, const float* __restrict__ dLdC, float* __restrict__ grad2d) {
+    const uint32_t* __restrict__ last, const float* __restrict__ dLdC, float* __restrict__ grad2d,
+    const uint32_t* __restrict__ order) {
     constexpr uint32_t SIDE = TS / 8, BLOCKS = SIDE * SIDE;
     __shared__ SplatB s_spl[4][64];
-    const uint32_t block = xcd_block(blockIdx.x, gridDim.x);
-    const uint32_t wv = block * 4 + (threadIdx.x >> 6);
-    const uint32_t tile = wv / BLOCKS, sub = wv % BLOCKS;
-    if (tile >= num_tiles) return;
+    const uint32_t wv = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t slot = wv / BLOCKS, sub = wv % BLOCKS;
+    if (slot >= num_tiles) return;
+    const uint32_t tile = order[slot];
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
     const uint32_t x0 = tx * TS + (sub % SIDE) * 8, y0 = ty * TS + (sub / SIDE) * 8;
@@ -182,7 +183,7 @@ __global__ __launch_bounds__(256) void k_raster_backward(
             const float dx = s.p.x - fx, dy = s.p.y - fy;
             const float q = __builtin_fmaf(A * dx, dx, (C * dy) * dy);
             const float power = __builtin_fmaf(-0.5f, q, -((B * dx) * dy));
-            const float e = exp_core(power);
+            const float e = exp_blend(power);
             const float oe = op * e;
             const float alpha = fminf(kAlphaClamp, oe);
             // blended in the forward: the forward's skip tests, as predicates
@@ -471,18 +472,21 @@ int launch_backward(Context* c, const float* params, uint32_t count, const CamK&
     mark(c, "backward raster");
     const uint32_t tiles = tx * ty;
     const uint32_t blocks = ceil_div(tiles * (tile * tile / 64), 4u);
+    uint32_t* order = (uint32_t*)scratch(c, "backward_tile_order", (size_t)tiles * 4);
+    if (!order) return GSR_ERR_ALLOC;
+    if ((rc = launch_tile_order(c, ranges, tiles, order))) return rc;
     switch (tile) {
         case 8:
             hipLaunchKernelGGL(k_raster_backward<8>, dim3(blocks), dim3(256), 0, c->stream, W, H, tx, tiles, count,
-                               &counters->n, ranges, vals, conic, xy, rgb, tfinal, last, dLdC, g2);
+                               &counters->n, ranges, vals, conic, xy, rgb, tfinal, last, dLdC, g2, order);
             break;
         case 16:
             hipLaunchKernelGGL(k_raster_backward<16>, dim3(blocks), dim3(256), 0, c->stream, W, H, tx, tiles, count,
-                               &counters->n, ranges, vals, conic, xy, rgb, tfinal, last, dLdC, g2);
+                               &counters->n, ranges, vals, conic, xy, rgb, tfinal, last, dLdC, g2, order);
             break;
         case 32:
             hipLaunchKernelGGL(k_raster_backward<32>, dim3(blocks), dim3(256), 0, c->stream, W, H, tx, tiles, count,
-                               &counters->n, ranges, vals, conic, xy, rgb, tfinal, last, dLdC, g2);
+                               &counters->n, ranges, vals, conic, xy, rgb, tfinal, last, dLdC, g2, order);
             break;
         default:
             return fail(c, GSR_ERR_STATE, "bad saved tile size %u", tile);

@@ -45,6 +45,26 @@ __device__ __forceinline__ float exp_core(float x) {
     return __builtin_ldexpf(p, (int)n);
 }
 
+// e^p for the alpha of a splat at a pixel, p = the power in [-5.55, 0] (the
+// range where alpha can reach 1/255; results outside it are never used):
+// 2^(p log2 e) with the integer part k rounded off against 1.5 * 2^23, the
+// fraction f = p log2 e - k (|f| <= 1/2) from one fused multiply-add, 2^f by a
+// degree-5 polynomial fitted for relative error on [-1/2, 1/2], scaled by 2^k.
+// Within 2.7 ulp of e^p for every float in [-5.55, 0].  The oracle's
+// exp_blend_f32 is the same sequence.
+__device__ __forceinline__ float exp_blend(float p) {
+    constexpr float kLog2e = 0x1.715476p+0f, kRound = 0x1.8p+23f;
+    const float t = __builtin_fmaf(p, kLog2e, kRound);
+    const float f = __builtin_fmaf(p, kLog2e, -(t - kRound));
+    float q = 0x1.5bba1p-10f;
+    q = __builtin_fmaf(q, f, 0x1.3cea88p-7f);
+    q = __builtin_fmaf(q, f, 0x1.c6b752p-5f);
+    q = __builtin_fmaf(q, f, 0x1.ebf9bcp-3f);
+    q = __builtin_fmaf(q, f, 0x1.62e42ap-1f);
+    q = __builtin_fmaf(q, f, 1.0f);
+    return __builtin_ldexpf(q, (int)(__float_as_uint(t) - __float_as_uint(kRound)));
+}
+
 __device__ __forceinline__ float gsr_exp(float x) {
     if (x != x) return x;
     if (x > 88.8f) return __builtin_inff();
@@ -195,6 +215,8 @@ int launch_raster(Context* c, uint32_t W, uint32_t H, uint32_t tile, uint32_t sp
                   uint32_t n_host, const uint32_t* ranges, const uint32_t* vals, const float4* conic,
                   const float2* xy, const float4* rgb, float* out, float* tfinal, uint32_t* last);
 int launch_fill_u32(Context* c, uint32_t* p, uint32_t value, size_t count);
+// order[0..tiles) = the tiles, heaviest (longest instance range) first.
+int launch_tile_order(Context* c, const uint2* ranges, uint32_t tiles, uint32_t* order);
 // Zeroes up to 4 regions (pointer, u32 count) with one launch.
 struct FillRegion {
     uint32_t* p;

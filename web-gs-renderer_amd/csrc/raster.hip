@@ -97,6 +97,8 @@ struct SplatL {
 // lane: lane l owns pixel (x0 + l % 8, y0 + l / 8).  start/end delimit the
 // tile's instances; out/tfinal/last address the view's image ([3][H][W]).
 // The caller's workgroup has 4 waves (wave w stages its batch in s_spl[w]).
+// SAVE: also write T_final and the last contributor (tfinal/last non-null).
+template <bool SAVE>
 __device__ __forceinline__ void raster_block(uint32_t W, uint32_t H, uint32_t x0, uint32_t y0, uint32_t start,
                                             uint32_t end, uint32_t splats, const uint32_t* __restrict__ vals,
                                             const float4* __restrict__ conic, const float2* __restrict__ xy,
@@ -147,7 +149,7 @@ __device__ __forceinline__ void raster_block(uint32_t W, uint32_t H, uint32_t x0
             const float dx = s.p.x - fx, dy = s.p.y - fy;
             const float q = __builtin_fmaf(s.co.x * dx, dx, (s.co.z * dy) * dy);
             const float power = __builtin_fmaf(-0.5f, q, -((s.co.y * dx) * dy));
-            const float alpha = fminf(kAlphaClamp, s.co.w * exp_core(power));
+            const float alpha = fminf(kAlphaClamp, s.co.w * exp_blend(power));
             const float tn = T * (1.0f - alpha);
             const bool ok = !done & !(power > 0.0f) & !(power < -5.55f) & !(alpha < kAlphaMin);
             const bool low = tn < kTransmitMin;
@@ -158,7 +160,7 @@ __device__ __forceinline__ void raster_block(uint32_t W, uint32_t H, uint32_t x0
                 c1 = __builtin_fmaf(s.cl.y, w, c1);
                 c2 = __builtin_fmaf(s.cl.z, w, c2);
                 T = tn;
-                lst = b - start + (uint32_t)j + 1;
+                if (SAVE) lst = b - start + (uint32_t)j + 1;
             }
         }
     }
@@ -168,19 +170,52 @@ __device__ __forceinline__ void raster_block(uint32_t W, uint32_t H, uint32_t x0
         out[o] = c0;
         out[plane + o] = c1;
         out[2 * plane + o] = c2;
-        if (tfinal) tfinal[o] = T;
-        if (last) last[o] = lst;
+        if (SAVE) {
+            tfinal[o] = T;
+            last[o] = lst;
+        }
+    }
+}
+
+// Heaviest tiles first: the raster kernels take tiles in the order written
+// here, tiles bucketed by instance count on a log scale (4 classes per
+// octave) and the classes in descending order, so the longest tiles start
+// first and the short ones fill in behind them instead of leaving a tail.
+// One workgroup: its LDS holds the 128 class counters.
+__global__ __launch_bounds__(1024) void k_tile_order(const uint2* __restrict__ ranges, uint32_t tiles,
+                                                     uint32_t* __restrict__ order) {
+    __shared__ uint32_t s_cnt[128];
+    const uint32_t tid = threadIdx.x;
+    if (tid < 128) s_cnt[tid] = 0;
+    __syncthreads();
+    auto cls = [](uint32_t c) -> uint32_t {
+        if (c < 4) return c;
+        const uint32_t lg = 31 - __builtin_clz(c);
+        return min(127u, 4 * lg + ((c >> (lg - 2)) & 3u) - 4);
+    };
+    for (uint32_t t = tid; t < tiles; t += 1024) {
+        const uint2 r = ranges[t];
+        atomicAdd(&s_cnt[127 - cls(r.y - r.x)], 1u);
+    }
+    __syncthreads();
+    if (tid < 64) {
+        const uint32_t a = s_cnt[2 * tid], b = s_cnt[2 * tid + 1];
+        const uint32_t incl = wave_inclusive_sum(a + b);
+        s_cnt[2 * tid] = incl - a - b;
+        s_cnt[2 * tid + 1] = incl - b;
+    }
+    __syncthreads();
+    for (uint32_t t = tid; t < tiles; t += 1024) {
+        const uint2 r = ranges[t];
+        order[atomicAdd(&s_cnt[127 - cls(r.y - r.x)], 1u)] = t;
     }
 }
 
 // Workgroups of 4 waves.  Wave w of the grid takes 8x8 block w % BLOCKS of
-// tile w / BLOCKS (blocks of a tile in row order), so a tile's blocks (and,
-// for 8-pixel tiles, neighbouring tiles) share a CU.  An 8x8 block is the
-// most compact 64-pixel footprint, which culls the most splats per wave.
-// Workgroups reach the 8 XCDs round-robin; workgroup ids are remapped so that
-// each XCD rasterizes one contiguous run of tiles and the splats they share
-// stay in its L2.
-template <int TS>
+// tile order[w / BLOCKS] (blocks of a tile in row order), so a tile's blocks
+// share a CU.  An 8x8 block is the most compact 64-pixel footprint, which
+// culls the most splats per wave.
+template <int TS, bool SAVE>
 __global__ __launch_bounds__(256) void k_raster(uint32_t W, uint32_t H, uint32_t tiles_x, uint32_t num_tiles,
                                                 uint32_t splats, const uint32_t* n_dev, uint32_t n_host,
                                                 const uint2* __restrict__ ranges,
@@ -188,18 +223,19 @@ __global__ __launch_bounds__(256) void k_raster(uint32_t W, uint32_t H, uint32_t
                                                 const float4* __restrict__ conic,
                                                 const float2* __restrict__ xy,
                                                 const float4* __restrict__ rgb, float* __restrict__ out,
-                                                float* __restrict__ tfinal, uint32_t* __restrict__ last) {
+                                                float* __restrict__ tfinal, uint32_t* __restrict__ last,
+                                                const uint32_t* __restrict__ order) {
     constexpr uint32_t SIDE = TS / 8, BLOCKS = SIDE * SIDE;
-    const uint32_t block = xcd_block(blockIdx.x, gridDim.x);
-    const uint32_t wv = block * 4 + (threadIdx.x >> 6);
-    const uint32_t tile = wv / BLOCKS, sub = wv % BLOCKS;
-    if (tile >= num_tiles) return;
+    const uint32_t wv = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t slot = wv / BLOCKS, sub = wv % BLOCKS;
+    if (slot >= num_tiles) return;
+    const uint32_t tile = order[slot];
     const uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
     const uint2 r = ranges[tile];
     const uint32_t n_inst = n_dev ? *n_dev : n_host;
     const uint32_t end = min(r.y, n_inst);  // ranges never exceed n; bounds the reads
-    raster_block(W, H, tx * TS + (sub % SIDE) * 8, ty * TS + (sub / SIDE) * 8, r.x, end, splats, vals, conic, xy, rgb,
-                 out, tfinal, last);
+    raster_block<SAVE>(W, H, tx * TS + (sub % SIDE) * 8, ty * TS + (sub / SIDE) * 8, r.x, end, splats, vals, conic, xy,
+                       rgb, out, tfinal, last);
 }
 
 // Persistent form for view batches.  The work items are the 8x8 blocks of
@@ -241,8 +277,8 @@ __global__ __launch_bounds__(256) void k_raster_queue(const uint4* __restrict__ 
         const uint32_t local = t - vi.w;
         const uint32_t tx = local % vi.z, ty = local / vi.z;
         const uint2 r = ranges[t];
-        raster_block(vi.x, vi.y, tx * TS + (sub % SIDE) * 8, ty * TS + (sub / SIDE) * 8, r.x, min(r.y, n_inst), splats,
-                     vals, conic, xy, rgb, out + pix[lo], nullptr, nullptr);
+        raster_block<false>(vi.x, vi.y, tx * TS + (sub % SIDE) * 8, ty * TS + (sub / SIDE) * 8, r.x,
+                            min(r.y, n_inst), splats, vals, conic, xy, rgb, out + pix[lo], nullptr, nullptr);
     }
 }
 
@@ -252,6 +288,13 @@ int launch_fill_u32(Context* c, uint32_t* p, uint32_t value, size_t count) {
     if (count == 0) return GSR_OK;
     const uint32_t blocks = (uint32_t)std::min<size_t>((count + 255) / 256, 2048);
     hipLaunchKernelGGL(k_fill_u32, dim3(blocks), dim3(256), 0, c->stream, p, value, count);
+    GSR_HIP(c, hipGetLastError());
+    return GSR_OK;
+}
+
+int launch_tile_order(Context* c, const uint2* ranges, uint32_t tiles, uint32_t* order) {
+    if (tiles == 0) return GSR_OK;
+    hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(1024), 0, c->stream, ranges, tiles, order);
     GSR_HIP(c, hipGetLastError());
     return GSR_OK;
 }
@@ -319,26 +362,51 @@ int launch_raster_queue(Context* c, uint32_t tile, const uint4* views, const uin
     return GSR_OK;
 }
 
+template <int TS>
+static void launch_raster_tile(Context* c, uint32_t blocks, uint32_t W, uint32_t H, uint32_t tx, uint32_t tiles,
+                               uint32_t splats, const uint32_t* n_dev, uint32_t n_host, const uint2* rg,
+                               const uint32_t* vals, const float4* conic, const float2* xy, const float4* rgb,
+                               float* out, float* tfinal, uint32_t* last, const uint32_t* order) {
+    if (tfinal && last)
+        hipLaunchKernelGGL((k_raster<TS, true>), dim3(blocks), dim3(256), 0, c->stream, W, H, tx, tiles, splats, n_dev,
+                           n_host, rg, vals, conic, xy, rgb, out, tfinal, last, order);
+    else
+        hipLaunchKernelGGL((k_raster<TS, false>), dim3(blocks), dim3(256), 0, c->stream, W, H, tx, tiles, splats, n_dev,
+                           n_host, rg, vals, conic, xy, rgb, out, nullptr, nullptr, order);
+}
+
+// T_final and the last contributor: when only one is requested the other goes
+// to a scratch buffer (one kernel writes both or neither).
 int launch_raster(Context* c, uint32_t W, uint32_t H, uint32_t tile, uint32_t splats, const uint32_t* n_dev,
                   uint32_t n_host, const uint32_t* ranges, const uint32_t* vals, const float4* conic,
                   const float2* xy, const float4* rgb, float* out, float* tfinal, uint32_t* last) {
     const uint32_t tx = ceil_div(W, tile), ty = ceil_div(H, tile);
     const uint32_t tiles = tx * ty;
     if (tiles == 0) return GSR_OK;
+    if ((tfinal != nullptr) != (last != nullptr)) {
+        void* tmp = scratch(c, "raster_state", (size_t)W * H * 4);
+        if (!tmp) return GSR_ERR_ALLOC;
+        if (!tfinal) tfinal = (float*)tmp;
+        else last = (uint32_t*)tmp;
+    }
     const uint2* rg = (const uint2*)ranges;
     const uint32_t blocks = ceil_div(tiles * (tile * tile / 64), 4u);
+    uint32_t* order = (uint32_t*)scratch(c, "tile_order", (size_t)tiles * 4);
+    if (!order) return GSR_ERR_ALLOC;
+    int rc = launch_tile_order(c, rg, tiles, order);
+    if (rc) return rc;
     switch (tile) {
         case 8:
-            hipLaunchKernelGGL(k_raster<8>, dim3(blocks), dim3(256), 0, c->stream, W, H, tx, tiles, splats, n_dev,
-                               n_host, rg, vals, conic, xy, rgb, out, tfinal, last);
+            launch_raster_tile<8>(c, blocks, W, H, tx, tiles, splats, n_dev, n_host, rg, vals, conic, xy, rgb, out,
+                                  tfinal, last, order);
             break;
         case 16:
-            hipLaunchKernelGGL(k_raster<16>, dim3(blocks), dim3(256), 0, c->stream, W, H, tx, tiles, splats, n_dev,
-                               n_host, rg, vals, conic, xy, rgb, out, tfinal, last);
+            launch_raster_tile<16>(c, blocks, W, H, tx, tiles, splats, n_dev, n_host, rg, vals, conic, xy, rgb, out,
+                                   tfinal, last, order);
             break;
         case 32:
-            hipLaunchKernelGGL(k_raster<32>, dim3(blocks), dim3(256), 0, c->stream, W, H, tx, tiles, splats, n_dev,
-                               n_host, rg, vals, conic, xy, rgb, out, tfinal, last);
+            launch_raster_tile<32>(c, blocks, W, H, tx, tiles, splats, n_dev, n_host, rg, vals, conic, xy, rgb, out,
+                                   tfinal, last, order);
             break;
         default:
             return fail(c, GSR_ERR_ARGUMENT, "tile size %u not in {8, 16, 32}", tile);

@@ -58,14 +58,4 @@ __device__ __forceinline__ void st_release_u64(uint64_t* p, uint64_t v) {
 // through the frame's error counter instead of hanging the GPU.
 constexpr uint32_t kSpinLimit = 1u << 22;
 
-// Workgroups of a grid reach the 8 XCDs round-robin by id (id % 8), and each
-// XCD has its own L2.  xcd_block maps a dispatch id b of an nb-block grid to a
-// logical id such that every XCD receives one contiguous range of logical ids
-// (a bijection on [0, nb)), so neighbouring work shares an L2.
-__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
-    constexpr uint32_t kXcds = 8;
-    const uint32_t q = nb / kXcds, r = nb % kXcds, x = b % kXcds, s = b / kXcds;
-    return x * q + min(x, r) + s;
-}
-
 }  // namespace gsr
