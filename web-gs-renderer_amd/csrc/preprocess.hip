@@ -6,6 +6,9 @@
 // 256-byte wave access.  Spherical harmonics (180 of the 236 bytes) are only
 // read for splats that survive both culls.  FP32 operation order is the
 // oracle's (no contraction, fmaf where written, IEEE sqrt/div, gsr_exp).
+// The view-independent part (covariance, opacity) is split from the
+// per-camera part so that view batches compute it once per splat; the
+// arithmetic and its order are the same either way.
 #include "gsr_internal.h"
 
 namespace gsr {
@@ -21,11 +24,54 @@ constexpr float SH_C3_0 = -0.5900435899266435f, SH_C3_1 = 2.890611442640554f,
                 SH_C3_4 = -0.4570457994644658f, SH_C3_5 = 1.445305721320277f,
                 SH_C3_6 = -0.5900435899266435f;
 
-// One splat seen by one camera.  Outputs as gsr_preprocess documents them.
-__device__ __forceinline__ void preprocess_splat(const float* __restrict__ P, size_t N, uint32_t g, const CamK& cam,
-                                                 uint32_t tile, uint32_t& ntiles, uint4& meta, float4& conic,
-                                                 float2& xy, float4& rgb) {
-    const float px = P[CH_POS * N + g], py = P[(CH_POS + 1) * N + g], pz = P[(CH_POS + 2) * N + g];
+// The view-independent part of a splat: position, 3-D covariance
+// Sigma = (R diag(s)) (R diag(s))^T (quantToMat3 / scaleToMat3,
+// preprocess.wgsl:173-201; symmetric, so six entries) and opacity.
+struct SplatFixed {
+    float px, py, pz;
+    float S00, S01, S02, S11, S12, S22;
+    float opacity;
+};
+
+__device__ __forceinline__ SplatFixed splat_fixed(const float* __restrict__ P, size_t N, uint32_t g) {
+    SplatFixed f;
+    f.px = P[CH_POS * N + g];
+    f.py = P[(CH_POS + 1) * N + g];
+    f.pz = P[(CH_POS + 2) * N + g];
+    const float s0 = gsr_exp(P[(CH_SCALE + 0) * N + g]);
+    const float s1 = gsr_exp(P[(CH_SCALE + 1) * N + g]);
+    const float s2 = gsr_exp(P[(CH_SCALE + 2) * N + g]);
+    float qw = P[(CH_ROT + 0) * N + g], qx = P[(CH_ROT + 1) * N + g];
+    float qy = P[(CH_ROT + 2) * N + g], qz = P[(CH_ROT + 3) * N + g];
+    const float ql = sqrtf(((qx * qx + qy * qy) + qz * qz) + qw * qw);
+    qx = qx / ql; qy = qy / ql; qz = qz / ql; qw = qw / ql;
+    const float x2 = qx + qx, y2 = qy + qy, z2 = qz + qz;
+    const float xx = qx * x2, xy_ = qx * y2, xz = qx * z2;
+    const float yy = qy * y2, yz = qy * z2, zz = qz * z2;
+    const float wx = qw * x2, wy = qw * y2, wz = qw * z2;
+    // R (row, col) as quantToMat3 (preprocess.wgsl:182-201), times diag(s)
+    const float m00 = (1.0f - (yy + zz)) * s0, m01 = (xy_ - wz) * s1, m02 = (xz + wy) * s2;
+    const float m10 = (xy_ + wz) * s0, m11 = (1.0f - (xx + zz)) * s1, m12 = (yz - wx) * s2;
+    const float m20 = (xz - wy) * s0, m21 = (yz + wx) * s1, m22 = (1.0f - (xx + yy)) * s2;
+    // Sigma = M M^T; the products commute, so Sij == Sji exactly
+    f.S00 = (m00 * m00 + m01 * m01) + m02 * m02;
+    f.S01 = (m00 * m10 + m01 * m11) + m02 * m12;
+    f.S02 = (m00 * m20 + m01 * m21) + m02 * m22;
+    f.S11 = (m10 * m10 + m11 * m11) + m12 * m12;
+    f.S12 = (m10 * m20 + m11 * m21) + m12 * m22;
+    f.S22 = (m20 * m20 + m21 * m21) + m22 * m22;
+    f.opacity = 1.0f / (1.0f + gsr_exp(-P[CH_OPACITY * N + g]));
+    return f;
+}
+
+// The splat seen by one camera.  sh(i) returns SH coefficient i of the
+// splat: i = ch for the DC terms, 3 + 15 ch + k for the rest.  Outputs as
+// gsr_preprocess documents them.
+template <typename SH>
+__device__ __forceinline__ void splat_view(const SplatFixed& f, SH sh, const CamK& cam, uint32_t tile,
+                                           uint32_t& ntiles, uint4& meta, float4& conic, float2& xy,
+                                           float4& rgb) {
+    const float px = f.px, py = f.py, pz = f.pz;
     const float* V = cam.view;
     const float* PV = cam.view_proj;
     const float tx = ((V[0] * px + V[4] * py) + V[8] * pz) + V[12];
@@ -47,31 +93,8 @@ __device__ __forceinline__ void preprocess_splat(const float* __restrict__ P, si
         const float pixx = ((ndcx + 1.0f) * W - 1.0f) * 0.5f;
         const float pixy = ((ndcy + 1.0f) * H - 1.0f) * 0.5f;
         xy = make_float2(pixx, pixy);
-        const float s0 = gsr_exp(P[(CH_SCALE + 0) * N + g]);
-        const float s1 = gsr_exp(P[(CH_SCALE + 1) * N + g]);
-        const float s2 = gsr_exp(P[(CH_SCALE + 2) * N + g]);
-        float qw = P[(CH_ROT + 0) * N + g], qx = P[(CH_ROT + 1) * N + g];
-        float qy = P[(CH_ROT + 2) * N + g], qz = P[(CH_ROT + 3) * N + g];
-        const float ql = sqrtf(((qx * qx + qy * qy) + qz * qz) + qw * qw);
-        qx = qx / ql; qy = qy / ql; qz = qz / ql; qw = qw / ql;
-        const float x2 = qx + qx, y2 = qy + qy, z2 = qz + qz;
-        const float xx = qx * x2, xy_ = qx * y2, xz = qx * z2;
-        const float yy = qy * y2, yz = qy * z2, zz = qz * z2;
-        const float wx = qw * x2, wy = qw * y2, wz = qw * z2;
-        // R (row, col) as quantToMat3 (preprocess.wgsl:182-201), times diag(s)
-        const float m00 = (1.0f - (yy + zz)) * s0, m01 = (xy_ - wz) * s1, m02 = (xz + wy) * s2;
-        const float m10 = (xy_ + wz) * s0, m11 = (1.0f - (xx + zz)) * s1, m12 = (yz - wx) * s2;
-        const float m20 = (xz - wy) * s0, m21 = (yz + wx) * s1, m22 = (1.0f - (xx + yy)) * s2;
-        // Sigma = M M^T (symmetric)
-        const float S00 = (m00 * m00 + m01 * m01) + m02 * m02;
-        const float S01 = (m00 * m10 + m01 * m11) + m02 * m12;
-        const float S02 = (m00 * m20 + m01 * m21) + m02 * m22;
-        const float S10 = (m10 * m00 + m11 * m01) + m12 * m02;
-        const float S11 = (m10 * m10 + m11 * m11) + m12 * m12;
-        const float S12 = (m10 * m20 + m11 * m21) + m12 * m22;
-        const float S20 = (m20 * m00 + m21 * m01) + m22 * m02;
-        const float S21 = (m20 * m10 + m21 * m11) + m22 * m12;
-        const float S22 = (m20 * m20 + m21 * m21) + m22 * m22;
+        const float S00 = f.S00, S01 = f.S01, S02 = f.S02, S11 = f.S11, S12 = f.S12, S22 = f.S22;
+        const float S10 = S01, S20 = S02, S21 = S12;
         // calculateCovariance2D (preprocess.wgsl:251-269)
         const float limx = kFrustumGuard * cam.tan_fov[0];
         const float limy = kFrustumGuard * cam.tan_fov[1];
@@ -97,8 +120,7 @@ __device__ __forceinline__ void preprocess_splat(const float* __restrict__ P, si
         const float mid = 0.5f * (a + c);
         const float lambda = mid + sqrtf(fmaxf(0.1f, mid * mid - det));
         const float radius = ceilf(3.0f * sqrtf(lambda));
-        conic = make_float4(c / det, -b / det, a / det,
-                            1.0f / (1.0f + gsr_exp(-P[CH_OPACITY * N + g])));
+        conic = make_float4(c / det, -b / det, a / det, f.opacity);
         if (!(pixx + radius < 0.0f || pixx - radius > W || pixy + radius < 0.0f || pixy - radius > H)) {
             // calculateColor (preprocess.wgsl:219-249)
             float dx = px - cam.eye[0], dy = py - cam.eye[1], dz = pz - cam.eye[2];
@@ -125,10 +147,9 @@ __device__ __forceinline__ void preprocess_splat(const float* __restrict__ P, si
             float col[3];
 #pragma unroll
             for (int ch = 0; ch < 3; ++ch) {
-                float acc = SH_C0 * P[(CH_DC + ch) * N + g];
+                float acc = SH_C0 * sh(ch);
 #pragma unroll
-                for (int k = 0; k < 15; ++k)
-                    acc = __builtin_fmaf(basis[k], P[(CH_REST + 15 * ch + k) * N + g], acc);
+                for (int k = 0; k < 15; ++k) acc = __builtin_fmaf(basis[k], sh(3 + 15 * ch + k), acc);
                 acc = acc + 0.5f;
                 col[ch] = fmaxf(acc, 0.0f);
             }
@@ -150,6 +171,14 @@ __device__ __forceinline__ void preprocess_splat(const float* __restrict__ P, si
     }
 }
 
+// SH coefficients straight from the planar scene.
+struct ShGlobal {
+    const float* __restrict__ P;
+    size_t N;
+    uint32_t g;
+    __device__ float operator()(int i) const { return P[(size_t)(i < 3 ? CH_DC + i : CH_REST + i - 3) * N + g]; }
+};
+
 __global__ __launch_bounds__(256) void k_preprocess(const float* __restrict__ P, uint32_t n, CamK cam,
                                                     uint32_t tile, uint32_t* __restrict__ tiles_out,
                                                     uint4* __restrict__ meta_out,
@@ -163,7 +192,7 @@ __global__ __launch_bounds__(256) void k_preprocess(const float* __restrict__ P,
     float4 conic;
     float2 xy;
     float4 rgb;
-    preprocess_splat(P, n, g, cam, tile, ntiles, meta, conic, xy, rgb);
+    splat_view(splat_fixed(P, n, g), ShGlobal{P, n, g}, cam, tile, ntiles, meta, conic, xy, rgb);
     tiles_out[g] = ntiles;
     meta_out[g] = meta;
     conic_out[g] = conic;
@@ -171,9 +200,19 @@ __global__ __launch_bounds__(256) void k_preprocess(const float* __restrict__ P,
     rgb_out[g] = rgb;
 }
 
-// View batches: one thread per splat walks every camera, so a splat's
-// parameters are fetched from HBM once for all views; outputs of view v sit
-// at v * n + g.
+// View batches: a workgroup takes 32 splats.  Their view-independent parts
+// (position, covariance, opacity) and SH coefficients are computed and staged
+// in LDS once; then thread (s, q) = (tid % 32, tid / 32) projects splat s for
+// views q, q + 8, q + 16, ...  Outputs of view v sit at v * n + g.
+constexpr int kBatchSplats = 32;
+constexpr int kBatchLanes = 8;  // view phases per splat
+
+struct ShShared {
+    const float* sh;  // [48][kBatchSplats]
+    uint32_t s;
+    __device__ float operator()(int i) const { return sh[i * kBatchSplats + s]; }
+};
+
 __global__ __launch_bounds__(256) void k_preprocess_batch(const float* __restrict__ P, uint32_t n,
                                                           const CamK* __restrict__ cams, uint32_t views,
                                                           uint32_t tile, uint32_t* __restrict__ tiles_out,
@@ -181,16 +220,29 @@ __global__ __launch_bounds__(256) void k_preprocess_batch(const float* __restric
                                                           float4* __restrict__ conic_out,
                                                           float2* __restrict__ xy_out,
                                                           float4* __restrict__ rgb_out) {
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= n) return;
-    for (uint32_t v = 0; v < views; ++v) {
-        const CamK cam = cams[v];
+    __shared__ float s_sh[48 * kBatchSplats];
+    __shared__ SplatFixed s_fixed[kBatchSplats];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t g0 = blockIdx.x * kBatchSplats;
+    const uint32_t count = min((uint32_t)kBatchSplats, n - g0);
+    for (uint32_t i = tid; i < 48 * kBatchSplats; i += blockDim.x) {
+        const uint32_t k = i / kBatchSplats, s = i % kBatchSplats;
+        const uint32_t ch = k < 3 ? CH_DC + k : CH_REST + k - 3;
+        s_sh[i] = s < count ? P[(size_t)ch * n + g0 + s] : 0.0f;
+    }
+    if (tid < count) s_fixed[tid] = splat_fixed(P, n, g0 + tid);
+    __syncthreads();
+    const uint32_t s = tid % kBatchSplats, q = tid / kBatchSplats;
+    if (s >= count) return;
+    const uint32_t g = g0 + s;
+    const SplatFixed f = s_fixed[s];
+    for (uint32_t v = q; v < views; v += kBatchLanes) {
         uint32_t ntiles;
         uint4 meta;
         float4 conic;
         float2 xy;
         float4 rgb;
-        preprocess_splat(P, n, g, cam, tile, ntiles, meta, conic, xy, rgb);
+        splat_view(f, ShShared{s_sh, s}, cams[v], tile, ntiles, meta, conic, xy, rgb);
         const size_t o = (size_t)v * n + g;
         tiles_out[o] = ntiles;
         meta_out[o] = meta;
@@ -205,8 +257,8 @@ __global__ __launch_bounds__(256) void k_preprocess_batch(const float* __restric
 int launch_preprocess_batch(Context* c, const float* params, uint32_t n, const CamK* cams, uint32_t views,
                             uint32_t tile, uint32_t* tiles, uint4* meta, float4* conic, float2* xy, float4* rgb) {
     if (n == 0 || views == 0) return GSR_OK;
-    hipLaunchKernelGGL(k_preprocess_batch, dim3(ceil_div(n, 256)), dim3(256), 0, c->stream, params, n, cams, views,
-                       tile, tiles, meta, conic, xy, rgb);
+    hipLaunchKernelGGL(k_preprocess_batch, dim3(ceil_div(n, kBatchSplats)), dim3(kBatchSplats * kBatchLanes), 0,
+                       c->stream, params, n, cams, views, tile, tiles, meta, conic, xy, rgb);
     GSR_HIP(c, hipGetLastError());
     return GSR_OK;
 }
