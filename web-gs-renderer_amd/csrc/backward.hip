@@ -111,8 +111,8 @@ struct SplatB {
 template <int TS>
 __global__ __launch_bounds__(256) void k_raster_backward(
     uint32_t W, uint32_t H, uint32_t tiles_x, uint32_t num_tiles, uint32_t splats, const uint32_t* n_dev,
-    const uint2* __restrict__ ranges, const uint32_t* __restrict__ vals, const float4* __restrict__ conic,
-    const float2* __restrict__ xy, const float4* __restrict__ rgb, const float* __restrict__ tfinal,
+    const uint2* __restrict__ ranges, const uint32_t* __restrict__ vals, const SplatRec* __restrict__ rec,
+    const float* __restrict__ tfinal,
     const uint32_t* __restrict__ last, const float* __restrict__ dLdC, float* __restrict__ grad2d,
     const uint32_t* __restrict__ order) {
     constexpr uint32_t SIDE = TS / 8, BLOCKS = SIDE * SIDE;
@@ -158,10 +158,11 @@ __global__ __launch_bounds__(256) void k_raster_backward(
             const uint32_t g = vals[idx];
             if (g < splats) {
                 gid = g;
-                co = conic[g];
-                p = xy[g];
-                cl = rgb[g];
-                live = reaches_rect(p.x, p.y, co.x, co.y, co.z, reach_qmax(co.w), rxa, rxb, rya, ryb);
+                const SplatRec sr = rec[g];
+                co = sr.co;
+                p = make_float2(sr.pq.x, sr.pq.y);
+                cl = sr.cl;
+                live = reaches_rect(p.x, p.y, co.x, co.y, co.z, sr.pq.z, rxa, rxb, rya, ryb);
             }
         }
         // Stage the batch in this wave's LDS slots (as in the forward).
@@ -238,7 +239,7 @@ constexpr float SH_C3_0 = -0.5900435899266435f, SH_C3_1 = 2.890611442640554f,
 // d/dy = -(C My + B Mx), d/dA = -Mxx / 2, d/dB = -Mxy, d/dC = -Myy / 2, with
 // the splat's forward conic (A, B, C).
 __global__ __launch_bounds__(256) void k_preprocess_backward(const float* __restrict__ P, uint32_t n, CamK cam,
-                                                             const float4* __restrict__ conic,
+                                                             const SplatRec* __restrict__ rec,
                                                              float4* __restrict__ grad2d,
                                                              float* __restrict__ grad) {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
@@ -253,7 +254,7 @@ __global__ __launch_bounds__(256) void k_preprocess_backward(const float* __rest
     }
     float Gv[9] = {0.f, 0.f, 0.f, 0.f, 0.f, G1.y, G1.z, G1.w, G2.x};
     if (any) {
-        const float4 cn = conic[g];
+        const float4 cn = rec[g].co;
         Gv[0] = -__builtin_fmaf(cn.x, G0.x, cn.y * G0.y);
         Gv[1] = -__builtin_fmaf(cn.z, G0.y, cn.y * G0.x);
         Gv[2] = -0.5f * G0.z;
@@ -463,9 +464,7 @@ int launch_backward(Context* c, const float* params, uint32_t count, const CamK&
     if (rc) return rc;
     const uint32_t* vals = (const uint32_t*)buffer(c, c->state_sorted_index ? "vals1" : "vals0", 0);
     const uint2* ranges = (const uint2*)buffer(c, "ranges", 0);
-    const float4* conic = (const float4*)buffer(c, "conic", 0);
-    const float2* xy = (const float2*)buffer(c, "xy", 0);
-    const float4* rgb = (const float4*)buffer(c, "rgb", 0);
+    const SplatRec* rec = (const SplatRec*)buffer(c, "splat_rec", 0);
     const float* tfinal = (const float*)buffer(c, "tfinal", 0);
     const uint32_t* last = (const uint32_t*)buffer(c, "last", 0);
     const FrameCounters* counters = (const FrameCounters*)buffer(c, "counters", 0);
@@ -478,15 +477,15 @@ int launch_backward(Context* c, const float* params, uint32_t count, const CamK&
     switch (tile) {
         case 8:
             hipLaunchKernelGGL(k_raster_backward<8>, dim3(blocks), dim3(256), 0, c->stream, W, H, tx, tiles, count,
-                               &counters->n, ranges, vals, conic, xy, rgb, tfinal, last, dLdC, g2, order);
+                               &counters->n, ranges, vals, rec, tfinal, last, dLdC, g2, order);
             break;
         case 16:
             hipLaunchKernelGGL(k_raster_backward<16>, dim3(blocks), dim3(256), 0, c->stream, W, H, tx, tiles, count,
-                               &counters->n, ranges, vals, conic, xy, rgb, tfinal, last, dLdC, g2, order);
+                               &counters->n, ranges, vals, rec, tfinal, last, dLdC, g2, order);
             break;
         case 32:
             hipLaunchKernelGGL(k_raster_backward<32>, dim3(blocks), dim3(256), 0, c->stream, W, H, tx, tiles, count,
-                               &counters->n, ranges, vals, conic, xy, rgb, tfinal, last, dLdC, g2, order);
+                               &counters->n, ranges, vals, rec, tfinal, last, dLdC, g2, order);
             break;
         default:
             return fail(c, GSR_ERR_STATE, "bad saved tile size %u", tile);
@@ -494,7 +493,7 @@ int launch_backward(Context* c, const float* params, uint32_t count, const CamK&
     GSR_HIP(c, hipGetLastError());
     mark(c, "backward preprocess");
     hipLaunchKernelGGL(k_preprocess_backward, dim3(ceil_div(count, 256)), dim3(256), 0, c->stream, params, count, cam,
-                       conic, (float4*)g2, grad);
+                       rec, (float4*)g2, grad);
     GSR_HIP(c, hipGetLastError());
     mark_end(c, false);
     if (grad2d_out) GSR_HIP(c, hipMemcpyAsync(grad2d_out, g2, N * 48, hipMemcpyDeviceToDevice, c->stream));

@@ -119,16 +119,14 @@ static int enqueue_batch(Context* c, const float* params, uint32_t count, const 
     uint32_t* tiles = (uint32_t*)buffer(c, "b_tiles", NV * 4);
     uint32_t* offsets = (uint32_t*)buffer(c, "b_offsets", (NV + 1) * 4);
     uint4* meta = (uint4*)buffer(c, "b_meta", NV * 16);
-    float4* conic = (float4*)buffer(c, "b_conic", NV * 16);
-    float2* xy = (float2*)buffer(c, "b_xy", NV * 8);
-    float4* rgb = (float4*)buffer(c, "b_rgb", NV * 16);
+    SplatRec* rec = (SplatRec*)buffer(c, "b_splat_rec", NV * sizeof(SplatRec));
     void* k0 = buffer(c, "b_keys0", p.capacity * p.key_bytes);
     void* k1 = buffer(c, "b_keys1", p.capacity * p.key_bytes);
     uint32_t* v0 = (uint32_t*)buffer(c, "b_vals0", p.capacity * 4);
     uint32_t* v1 = (uint32_t*)buffer(c, "b_vals1", p.capacity * 4);
     uint32_t* ranges = (uint32_t*)buffer(c, "b_ranges", (size_t)p.total_tiles * 8);
     FrameCounters* counters = (FrameCounters*)buffer(c, "b_counters", sizeof(FrameCounters));
-    if (!tables || !tiles || !offsets || !meta || !conic || !xy || !rgb || !k0 || !k1 || !v0 || !v1 || !ranges ||
+    if (!tables || !tiles || !offsets || !meta || !rec || !k0 || !k1 || !v0 || !v1 || !ranges ||
         !counters)
         return GSR_ERR_ALLOC;
     c->scratch_prefix = "b_";
@@ -149,7 +147,7 @@ static int enqueue_batch(Context* c, const float* params, uint32_t count, const 
                                  {ranges, (size_t)p.total_tiles * 2}};
     if ((rc = launch_zero_regions(c, clear, 3))) return rc;
     mark(c, "preprocess");
-    if ((rc = launch_preprocess_batch(c, params, count, cams_dev, p.views, p.tile, tiles, meta, conic, xy, rgb)))
+    if ((rc = launch_preprocess_batch(c, params, count, cams_dev, p.views, p.tile, tiles, meta, rec)))
         return rc;
     mark(c, "offset scan");
     if ((rc = launch_scan(c, tiles, (uint32_t)NV, offsets, counters, p.capacity, true))) return rc;
@@ -167,7 +165,7 @@ static int enqueue_batch(Context* c, const float* params, uint32_t count, const 
     if ((rc = launch_ranges(c, ks, p.key_bytes, &counters->n, 0, p.total_tiles, ranges, false))) return rc;
     mark(c, "raster");
     if ((rc = launch_raster_queue(c, p.tile, info_dev, pix_dev, p.views, p.total_tiles, (uint32_t)NV, &counters->n,
-                                  &counters->raster_ticket, ranges, vs, conic, xy, rgb, out, p.raster_waves)))
+                                  &counters->raster_ticket, ranges, vs, rec, out, p.raster_waves)))
         return rc;
     mark_end(c);
     return GSR_OK;

@@ -72,6 +72,18 @@ __device__ __forceinline__ float gsr_exp(float x) {
     return exp_core(x);
 }
 
+// What the raster kernels gather per splat: 48 bytes, so that one instance
+// touches one or two 128-byte cache lines instead of three separate arrays'.
+//   co = (A, B, C, opacity): conic and opacity
+//   pq = (pixel x, pixel y, q_max, 0): q_max the widened reach threshold of
+//        cull.h, computed once per splat
+//   cl = (r, g, b, 0)
+// The record of a splat that touches no tile is never read.
+struct SplatRec {
+    float4 co, pq, cl;
+};
+static_assert(sizeof(SplatRec) == 48, "splat record must be 48 bytes");
+
 // Camera block as the kernels read it (same bytes as gsr_camera).
 struct CamK {
     float view[16];
@@ -188,10 +200,15 @@ void mark_end(Context* c, bool frame_end = true);
 void finish_frame_timings(Context* c);
 
 // ---- kernel launchers (each enqueues on c->stream) -------------------------
+// Preprocess writes either the three separate arrays (conic, xy, rgb; the
+// stage entry point) or, when rec is non-null, the packed records.
 int launch_preprocess(Context* c, const float* params, uint32_t n, const CamK& cam, uint32_t tile,
-                      uint32_t* tiles, uint4* meta, float4* conic, float2* xy, float4* rgb);
+                      uint32_t* tiles, uint4* meta, float4* conic, float2* xy, float4* rgb, SplatRec* rec);
 int launch_preprocess_batch(Context* c, const float* params, uint32_t n, const CamK* cams, uint32_t views,
-                            uint32_t tile, uint32_t* tiles, uint4* meta, float4* conic, float2* xy, float4* rgb);
+                            uint32_t tile, uint32_t* tiles, uint4* meta, SplatRec* rec);
+// Packs separate (conic, xy, rgb) arrays into records.
+int launch_pack_splats(Context* c, const float4* conic, const float2* xy, const float4* rgb, uint32_t n,
+                       SplatRec* rec);
 int launch_emit_batch(Context* c, const uint32_t* offsets, const uint4* meta, uint32_t n, uint32_t num_views,
                       const uint4* views, uint64_t capacity, uint32_t key_bytes, void* keys, uint32_t* vals);
 // Scan of n counts; its look-back status buffer (scan_status_buffer, of
@@ -210,10 +227,10 @@ int launch_sort(Context* c, uint32_t key_bytes, void* k0, uint32_t* v0, void* k1
                 uint32_t end_bit, uint32_t blocks_per_cu, FrameCounters* counters, int* result_index);
 int launch_ranges(Context* c, const void* keys, uint32_t key_bytes, const uint32_t* n_dev,
                   uint64_t n_host, uint32_t num_tiles, uint32_t* ranges, bool clear);
-// splats bounds the per-splat arrays, the instance count (n_dev, else n_host) the ranges.
+// splats bounds the per-splat records, the instance count (n_dev, else n_host) the ranges.
 int launch_raster(Context* c, uint32_t W, uint32_t H, uint32_t tile, uint32_t splats, const uint32_t* n_dev,
-                  uint32_t n_host, const uint32_t* ranges, const uint32_t* vals, const float4* conic,
-                  const float2* xy, const float4* rgb, float* out, float* tfinal, uint32_t* last);
+                  uint32_t n_host, const uint32_t* ranges, const uint32_t* vals, const SplatRec* rec, float* out,
+                  float* tfinal, uint32_t* last);
 int launch_fill_u32(Context* c, uint32_t* p, uint32_t value, size_t count);
 // order[0..tiles) = the tiles, heaviest (longest instance range) first.
 int launch_tile_order(Context* c, const uint2* ranges, uint32_t tiles, uint32_t* order);
@@ -226,8 +243,8 @@ int launch_zero_regions(Context* c, const FillRegion* regions, int count);
 // Batch raster: views[v] = {W, H, tiles_x, tile_base}, pix[v] = float offset of view v's image.
 int launch_raster_queue(Context* c, uint32_t tile, const uint4* views, const uint64_t* pix, uint32_t num_views,
                         uint32_t total_tiles, uint32_t splats, const uint32_t* n_dev, uint32_t* ticket,
-                        const uint32_t* ranges, const uint32_t* vals, const float4* conic, const float2* xy,
-                        const float4* rgb, float* out, uint32_t waves_per_cu);
+                        const uint32_t* ranges, const uint32_t* vals, const SplatRec* rec, float* out,
+                        uint32_t waves_per_cu);
 
 __host__ __device__ inline uint32_t ceil_div(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 

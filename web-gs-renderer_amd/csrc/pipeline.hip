@@ -61,9 +61,7 @@ int enqueue_forward(Context* c, const float* params, uint32_t count, const gsr_c
     uint32_t* tiles = (uint32_t*)buffer(c, "tiles", N * 4);
     uint32_t* offsets = (uint32_t*)buffer(c, "offsets", (N + 1) * 4);
     uint4* meta = (uint4*)buffer(c, "meta", N * 16);
-    float4* conic = (float4*)buffer(c, "conic", N * 16);
-    float2* xy = (float2*)buffer(c, "xy", N * 8);
-    float4* rgb = (float4*)buffer(c, "rgb", N * 16);
+    SplatRec* rec = (SplatRec*)buffer(c, "splat_rec", N * sizeof(SplatRec));
     void* k0 = buffer(c, "keys0", p.capacity * p.key_bytes);
     void* k1 = buffer(c, "keys1", p.capacity * p.key_bytes);
     uint32_t* v0 = (uint32_t*)buffer(c, "vals0", p.capacity * 4);
@@ -74,7 +72,7 @@ int enqueue_forward(Context* c, const float* params, uint32_t count, const gsr_c
     float* tfinal = save_state ? (float*)buffer(c, "tfinal", px * 4) : nullptr;
     uint32_t* last = save_state ? (uint32_t*)buffer(c, "last", px * 4) : nullptr;
     FrameCounters* counters = (FrameCounters*)buffer(c, "counters", sizeof(FrameCounters));
-    if (!tiles || !offsets || !meta || !conic || !xy || !rgb || !k0 || !k1 || !v0 || !v1 || !ranges || !image ||
+    if (!tiles || !offsets || !meta || !rec || !k0 || !k1 || !v0 || !v1 || !ranges || !image ||
         (save_state && (!tfinal || !last)) || !counters)
         return GSR_ERR_ALLOC;
     CamK ck;
@@ -89,7 +87,7 @@ int enqueue_forward(Context* c, const float* params, uint32_t count, const gsr_c
                                  {ranges, (size_t)p.num_tiles * 2}};
     if ((rc = launch_zero_regions(c, clear, 3))) return rc;
     mark(c, "preprocess");
-    if ((rc = launch_preprocess(c, params, count, ck, p.tile, tiles, meta, conic, xy, rgb))) return rc;
+    if ((rc = launch_preprocess(c, params, count, ck, p.tile, tiles, meta, nullptr, nullptr, nullptr, rec))) return rc;
     mark(c, "offset scan");
     if ((rc = launch_scan(c, tiles, count, offsets, counters, p.capacity, true))) return rc;
     mark(c, "emit");
@@ -104,7 +102,7 @@ int enqueue_forward(Context* c, const float* params, uint32_t count, const gsr_c
     uint32_t* vs = idx ? v1 : v0;
     if ((rc = launch_ranges(c, ks, p.key_bytes, &counters->n, 0, p.num_tiles, ranges, false))) return rc;
     mark(c, "raster");
-    if ((rc = launch_raster(c, p.W, p.H, p.tile, count, &counters->n, 0, ranges, vs, conic, xy, rgb, image, tfinal,
+    if ((rc = launch_raster(c, p.W, p.H, p.tile, count, &counters->n, 0, ranges, vs, rec, image, tfinal,
                              last)))
         return rc;
     mark_end(c);
@@ -213,7 +211,7 @@ int gsr_preprocess(gsr_context* ctx, const float* params_dev, uint32_t count, co
     std::memcpy(&ck, cam, sizeof(ck));
     ck.count = count;
     int rc = launch_preprocess(c, params_dev, count, ck, tile_size, tiles_touched_dev, (uint4*)meta_dev,
-                               (float4*)conic_opacity_dev, (float2*)xy_dev, (float4*)rgb_dev);
+                               (float4*)conic_opacity_dev, (float2*)xy_dev, (float4*)rgb_dev, nullptr);
     if (rc) return rc;
     GSR_HIP(c, hipStreamSynchronize(c->stream));
     return GSR_OK;
@@ -301,9 +299,13 @@ int gsr_rasterize(gsr_context* ctx, uint32_t width, uint32_t height, uint32_t ti
     if (width == 0 || height == 0) return fail(c, GSR_ERR_ARGUMENT, "empty image");
     hipSetDevice(c->device);
     if (instances > 0xFFFFFFFFull) return fail(c, GSR_ERR_ARGUMENT, "too many instances");
-    int rc = launch_raster(c, width, height, tile_size, count, nullptr, (uint32_t)instances, ranges_dev, values_dev,
-                           (const float4*)conic_opacity_dev, (const float2*)xy_dev, (const float4*)rgb_dev,
-                           out_rgb_dev, T_final_dev, last_dev);
+    SplatRec* rec = (SplatRec*)scratch(c, "stage_splat_rec", (size_t)std::max<uint32_t>(count, 1) * sizeof(SplatRec));
+    if (!rec) return GSR_ERR_ALLOC;
+    int rc = launch_pack_splats(c, (const float4*)conic_opacity_dev, (const float2*)xy_dev, (const float4*)rgb_dev,
+                                count, rec);
+    if (!rc)
+        rc = launch_raster(c, width, height, tile_size, count, nullptr, (uint32_t)instances, ranges_dev, values_dev,
+                           rec, out_rgb_dev, T_final_dev, last_dev);
     if (rc) return rc;
     GSR_HIP(c, hipStreamSynchronize(c->stream));
     return GSR_OK;

@@ -9,6 +9,7 @@
 // The view-independent part (covariance, opacity) is split from the
 // per-camera part so that view batches compute it once per splat; the
 // arithmetic and its order are the same either way.
+#include "cull.h"
 #include "gsr_internal.h"
 
 namespace gsr {
@@ -179,12 +180,22 @@ struct ShGlobal {
     __device__ float operator()(int i) const { return P[(size_t)(i < 3 ? CH_DC + i : CH_REST + i - 3) * N + g]; }
 };
 
+__device__ __forceinline__ SplatRec make_rec(const float4& conic, const float2& xy, const float4& rgb) {
+    SplatRec r;
+    r.co = conic;
+    r.pq = make_float4(xy.x, xy.y, reach_qmax(conic.w), 0.0f);
+    r.cl = rgb;
+    return r;
+}
+
+// PACKED: write records (rec_out) instead of the three arrays.
+template <bool PACKED>
 __global__ __launch_bounds__(256) void k_preprocess(const float* __restrict__ P, uint32_t n, CamK cam,
                                                     uint32_t tile, uint32_t* __restrict__ tiles_out,
                                                     uint4* __restrict__ meta_out,
                                                     float4* __restrict__ conic_out,
                                                     float2* __restrict__ xy_out,
-                                                    float4* __restrict__ rgb_out) {
+                                                    float4* __restrict__ rgb_out, SplatRec* __restrict__ rec_out) {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
     uint32_t ntiles;
@@ -195,9 +206,26 @@ __global__ __launch_bounds__(256) void k_preprocess(const float* __restrict__ P,
     splat_view(splat_fixed(P, n, g), ShGlobal{P, n, g}, cam, tile, ntiles, meta, conic, xy, rgb);
     tiles_out[g] = ntiles;
     meta_out[g] = meta;
-    conic_out[g] = conic;
-    xy_out[g] = xy;
-    rgb_out[g] = rgb;
+    if (PACKED) {
+        rec_out[g] = make_rec(conic, xy, rgb);
+    } else {
+        conic_out[g] = conic;
+        xy_out[g] = xy;
+        rgb_out[g] = rgb;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_pack(const float4* __restrict__ conic, const float2* __restrict__ xy,
+                                              const float4* __restrict__ rgb, uint32_t n,
+                                              SplatRec* __restrict__ rec) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    SplatRec r;
+    r.co = conic[g];
+    const float2 p = xy[g];
+    r.pq = make_float4(p.x, p.y, reach_qmax(r.co.w), 0.0f);
+    r.cl = rgb[g];
+    rec[g] = r;
 }
 
 // View batches: a workgroup takes 32 splats.  Their view-independent parts
@@ -217,9 +245,7 @@ __global__ __launch_bounds__(256) void k_preprocess_batch(const float* __restric
                                                           const CamK* __restrict__ cams, uint32_t views,
                                                           uint32_t tile, uint32_t* __restrict__ tiles_out,
                                                           uint4* __restrict__ meta_out,
-                                                          float4* __restrict__ conic_out,
-                                                          float2* __restrict__ xy_out,
-                                                          float4* __restrict__ rgb_out) {
+                                                          SplatRec* __restrict__ rec_out) {
     __shared__ float s_sh[48 * kBatchSplats];
     __shared__ SplatFixed s_fixed[kBatchSplats];
     const uint32_t tid = threadIdx.x;
@@ -246,28 +272,38 @@ __global__ __launch_bounds__(256) void k_preprocess_batch(const float* __restric
         const size_t o = (size_t)v * n + g;
         tiles_out[o] = ntiles;
         meta_out[o] = meta;
-        conic_out[o] = conic;
-        xy_out[o] = xy;
-        rgb_out[o] = rgb;
+        rec_out[o] = make_rec(conic, xy, rgb);
     }
 }
 
 }  // namespace
 
 int launch_preprocess_batch(Context* c, const float* params, uint32_t n, const CamK* cams, uint32_t views,
-                            uint32_t tile, uint32_t* tiles, uint4* meta, float4* conic, float2* xy, float4* rgb) {
+                            uint32_t tile, uint32_t* tiles, uint4* meta, SplatRec* rec) {
     if (n == 0 || views == 0) return GSR_OK;
     hipLaunchKernelGGL(k_preprocess_batch, dim3(ceil_div(n, kBatchSplats)), dim3(kBatchSplats * kBatchLanes), 0,
-                       c->stream, params, n, cams, views, tile, tiles, meta, conic, xy, rgb);
+                       c->stream, params, n, cams, views, tile, tiles, meta, rec);
     GSR_HIP(c, hipGetLastError());
     return GSR_OK;
 }
 
 int launch_preprocess(Context* c, const float* params, uint32_t n, const CamK& cam, uint32_t tile,
-                      uint32_t* tiles, uint4* meta, float4* conic, float2* xy, float4* rgb) {
+                      uint32_t* tiles, uint4* meta, float4* conic, float2* xy, float4* rgb, SplatRec* rec) {
     if (n == 0) return GSR_OK;
-    hipLaunchKernelGGL(k_preprocess, dim3(ceil_div(n, 256)), dim3(256), 0, c->stream, params, n, cam, tile,
-                       tiles, meta, conic, xy, rgb);
+    if (rec)
+        hipLaunchKernelGGL(k_preprocess<true>, dim3(ceil_div(n, 256)), dim3(256), 0, c->stream, params, n, cam, tile,
+                           tiles, meta, nullptr, nullptr, nullptr, rec);
+    else
+        hipLaunchKernelGGL(k_preprocess<false>, dim3(ceil_div(n, 256)), dim3(256), 0, c->stream, params, n, cam, tile,
+                           tiles, meta, conic, xy, rgb, nullptr);
+    GSR_HIP(c, hipGetLastError());
+    return GSR_OK;
+}
+
+int launch_pack_splats(Context* c, const float4* conic, const float2* xy, const float4* rgb, uint32_t n,
+                       SplatRec* rec) {
+    if (n == 0) return GSR_OK;
+    hipLaunchKernelGGL(k_pack, dim3(ceil_div(n, 256)), dim3(256), 0, c->stream, conic, xy, rgb, n, rec);
     GSR_HIP(c, hipGetLastError());
     return GSR_OK;
 }

@@ -101,8 +101,7 @@ struct SplatL {
 template <bool SAVE>
 __device__ __forceinline__ void raster_block(uint32_t W, uint32_t H, uint32_t x0, uint32_t y0, uint32_t start,
                                             uint32_t end, uint32_t splats, const uint32_t* __restrict__ vals,
-                                            const float4* __restrict__ conic, const float2* __restrict__ xy,
-                                            const float4* __restrict__ rgb, float* __restrict__ out,
+                                            const SplatRec* __restrict__ rec, float* __restrict__ out,
                                             float* __restrict__ tfinal, uint32_t* __restrict__ last) {
     __shared__ SplatL s_spl[4][64];
     const uint32_t lane = threadIdx.x & 63;
@@ -123,10 +122,11 @@ __device__ __forceinline__ void raster_block(uint32_t W, uint32_t H, uint32_t x0
         if (idx < end) {
             const uint32_t g = vals[idx];
             if (g < splats) {
-                co = conic[g];
-                p = xy[g];
-                cl = rgb[g];
-                live = reaches_rect(p.x, p.y, co.x, co.y, co.z, reach_qmax(co.w), rxa, rxb, rya, ryb);
+                const SplatRec sr = rec[g];
+                co = sr.co;
+                p = make_float2(sr.pq.x, sr.pq.y);
+                cl = sr.cl;
+                live = reaches_rect(p.x, p.y, co.x, co.y, co.z, sr.pq.z, rxa, rxb, rya, ryb);
             }
         }
         // Stage the batch in this wave's LDS slots.  Only this wave touches
@@ -220,9 +220,7 @@ __global__ __launch_bounds__(256) void k_raster(uint32_t W, uint32_t H, uint32_t
                                                 uint32_t splats, const uint32_t* n_dev, uint32_t n_host,
                                                 const uint2* __restrict__ ranges,
                                                 const uint32_t* __restrict__ vals,
-                                                const float4* __restrict__ conic,
-                                                const float2* __restrict__ xy,
-                                                const float4* __restrict__ rgb, float* __restrict__ out,
+                                                const SplatRec* __restrict__ rec, float* __restrict__ out,
                                                 float* __restrict__ tfinal, uint32_t* __restrict__ last,
                                                 const uint32_t* __restrict__ order) {
     constexpr uint32_t SIDE = TS / 8, BLOCKS = SIDE * SIDE;
@@ -234,8 +232,8 @@ __global__ __launch_bounds__(256) void k_raster(uint32_t W, uint32_t H, uint32_t
     const uint2 r = ranges[tile];
     const uint32_t n_inst = n_dev ? *n_dev : n_host;
     const uint32_t end = min(r.y, n_inst);  // ranges never exceed n; bounds the reads
-    raster_block<SAVE>(W, H, tx * TS + (sub % SIDE) * 8, ty * TS + (sub / SIDE) * 8, r.x, end, splats, vals, conic, xy,
-                       rgb, out, tfinal, last);
+    raster_block<SAVE>(W, H, tx * TS + (sub % SIDE) * 8, ty * TS + (sub / SIDE) * 8, r.x, end, splats, vals, rec,
+                       out, tfinal, last);
 }
 
 // Persistent form for view batches.  The work items are the 8x8 blocks of
@@ -249,9 +247,7 @@ __global__ __launch_bounds__(256) void k_raster_queue(const uint4* __restrict__ 
                                                       uint32_t total_tiles, uint32_t splats, const uint32_t* n_dev,
                                                       uint32_t* ticket, const uint2* __restrict__ ranges,
                                                       const uint32_t* __restrict__ vals,
-                                                      const float4* __restrict__ conic,
-                                                      const float2* __restrict__ xy, const float4* __restrict__ rgb,
-                                                      float* __restrict__ out) {
+                                                      const SplatRec* __restrict__ rec, float* __restrict__ out) {
     constexpr uint32_t SIDE = TS / 8, BLOCKS = SIDE * SIDE;
     __shared__ uint32_t s_ticket[2];
     const uint32_t n_inst = *n_dev;
@@ -278,7 +274,7 @@ __global__ __launch_bounds__(256) void k_raster_queue(const uint4* __restrict__ 
         const uint32_t tx = local % vi.z, ty = local / vi.z;
         const uint2 r = ranges[t];
         raster_block<false>(vi.x, vi.y, tx * TS + (sub % SIDE) * 8, ty * TS + (sub / SIDE) * 8, r.x,
-                            min(r.y, n_inst), splats, vals, conic, xy, rgb, out + pix[lo], nullptr, nullptr);
+                            min(r.y, n_inst), splats, vals, rec, out + pix[lo], nullptr, nullptr);
     }
 }
 
@@ -335,8 +331,8 @@ int launch_ranges(Context* c, const void* keys, uint32_t key_bytes, const uint32
 
 int launch_raster_queue(Context* c, uint32_t tile, const uint4* views, const uint64_t* pix, uint32_t num_views,
                         uint32_t total_tiles, uint32_t splats, const uint32_t* n_dev, uint32_t* ticket,
-                        const uint32_t* ranges, const uint32_t* vals, const float4* conic, const float2* xy,
-                        const float4* rgb, float* out, uint32_t waves_per_cu) {
+                        const uint32_t* ranges, const uint32_t* vals, const SplatRec* rec, float* out,
+                        uint32_t waves_per_cu) {
     if (total_tiles == 0) return GSR_OK;
     const uint32_t tickets = ceil_div(total_tiles * (tile * tile / 64), 4u);
     const uint32_t groups =
@@ -345,15 +341,15 @@ int launch_raster_queue(Context* c, uint32_t tile, const uint4* views, const uin
     switch (tile) {
         case 8:
             hipLaunchKernelGGL(k_raster_queue<8>, dim3(groups), dim3(256), 0, c->stream, views, pix, num_views,
-                               total_tiles, splats, n_dev, ticket, rg, vals, conic, xy, rgb, out);
+                               total_tiles, splats, n_dev, ticket, rg, vals, rec, out);
             break;
         case 16:
             hipLaunchKernelGGL(k_raster_queue<16>, dim3(groups), dim3(256), 0, c->stream, views, pix, num_views,
-                               total_tiles, splats, n_dev, ticket, rg, vals, conic, xy, rgb, out);
+                               total_tiles, splats, n_dev, ticket, rg, vals, rec, out);
             break;
         case 32:
             hipLaunchKernelGGL(k_raster_queue<32>, dim3(groups), dim3(256), 0, c->stream, views, pix, num_views,
-                               total_tiles, splats, n_dev, ticket, rg, vals, conic, xy, rgb, out);
+                               total_tiles, splats, n_dev, ticket, rg, vals, rec, out);
             break;
         default:
             return fail(c, GSR_ERR_ARGUMENT, "tile size %u not in {8, 16, 32}", tile);
@@ -365,21 +361,21 @@ int launch_raster_queue(Context* c, uint32_t tile, const uint4* views, const uin
 template <int TS>
 static void launch_raster_tile(Context* c, uint32_t blocks, uint32_t W, uint32_t H, uint32_t tx, uint32_t tiles,
                                uint32_t splats, const uint32_t* n_dev, uint32_t n_host, const uint2* rg,
-                               const uint32_t* vals, const float4* conic, const float2* xy, const float4* rgb,
-                               float* out, float* tfinal, uint32_t* last, const uint32_t* order) {
+                               const uint32_t* vals, const SplatRec* rec, float* out, float* tfinal, uint32_t* last,
+                               const uint32_t* order) {
     if (tfinal && last)
         hipLaunchKernelGGL((k_raster<TS, true>), dim3(blocks), dim3(256), 0, c->stream, W, H, tx, tiles, splats, n_dev,
-                           n_host, rg, vals, conic, xy, rgb, out, tfinal, last, order);
+                           n_host, rg, vals, rec, out, tfinal, last, order);
     else
         hipLaunchKernelGGL((k_raster<TS, false>), dim3(blocks), dim3(256), 0, c->stream, W, H, tx, tiles, splats, n_dev,
-                           n_host, rg, vals, conic, xy, rgb, out, nullptr, nullptr, order);
+                           n_host, rg, vals, rec, out, nullptr, nullptr, order);
 }
 
 // T_final and the last contributor: when only one is requested the other goes
 // to a scratch buffer (one kernel writes both or neither).
 int launch_raster(Context* c, uint32_t W, uint32_t H, uint32_t tile, uint32_t splats, const uint32_t* n_dev,
-                  uint32_t n_host, const uint32_t* ranges, const uint32_t* vals, const float4* conic,
-                  const float2* xy, const float4* rgb, float* out, float* tfinal, uint32_t* last) {
+                  uint32_t n_host, const uint32_t* ranges, const uint32_t* vals, const SplatRec* rec, float* out,
+                  float* tfinal, uint32_t* last) {
     const uint32_t tx = ceil_div(W, tile), ty = ceil_div(H, tile);
     const uint32_t tiles = tx * ty;
     if (tiles == 0) return GSR_OK;
@@ -397,16 +393,16 @@ int launch_raster(Context* c, uint32_t W, uint32_t H, uint32_t tile, uint32_t sp
     if (rc) return rc;
     switch (tile) {
         case 8:
-            launch_raster_tile<8>(c, blocks, W, H, tx, tiles, splats, n_dev, n_host, rg, vals, conic, xy, rgb, out,
-                                  tfinal, last, order);
+            launch_raster_tile<8>(c, blocks, W, H, tx, tiles, splats, n_dev, n_host, rg, vals, rec, out, tfinal,
+                                  last, order);
             break;
         case 16:
-            launch_raster_tile<16>(c, blocks, W, H, tx, tiles, splats, n_dev, n_host, rg, vals, conic, xy, rgb, out,
-                                   tfinal, last, order);
+            launch_raster_tile<16>(c, blocks, W, H, tx, tiles, splats, n_dev, n_host, rg, vals, rec, out, tfinal,
+                                   last, order);
             break;
         case 32:
-            launch_raster_tile<32>(c, blocks, W, H, tx, tiles, splats, n_dev, n_host, rg, vals, conic, xy, rgb, out,
-                                   tfinal, last, order);
+            launch_raster_tile<32>(c, blocks, W, H, tx, tiles, splats, n_dev, n_host, rg, vals, rec, out, tfinal,
+                                   last, order);
             break;
         default:
             return fail(c, GSR_ERR_ARGUMENT, "tile size %u not in {8, 16, 32}", tile);
