@@ -26,7 +26,7 @@ FP32_PEAK_TFLOPS = 157.3
 # HBM bytes per launch of the forward's kernels at this workload, from the
 # rocprofv3 FETCH_SIZE / WRITE_SIZE runs summarised in
 # profiles/forward_1m_1080p_pmc_traffic.md (calibrated against a plain copy).
-MEASURED_TRAFFIC = {"raster": 850439425, "preprocess": 296095744, "sort": 4 * 60755968 + 15953100}
+MEASURED_TRAFFIC = {"raster": 628783976, "preprocess": 304136736, "sort": 4 * 55914324 + 13873280 + 880032}
 
 
 def log(msg):
@@ -37,9 +37,9 @@ def algorithmic_bytes(stage: str, n_splats: int, instances: int, visible: int, w
                       tiles: int, key_bytes: int, sort_passes: int) -> float:
     """Algorithmic HBM bytes per launch of each stage: what the stage must read and write at least."""
     if stage == "preprocess":
-        # read 236 B of parameters for a visible splat (56 B when culled before SH),
-        # write tiles 4 + meta 16 + conic 16 + xy 8 + rgb 16
-        return visible * 236 + (n_splats - visible) * 56 + n_splats * 60
+        # read 236 B of parameters for a visible splat (44 B: position, scale, rotation,
+        # opacity when culled before the SH), write tiles 4 + meta 16 + splat record 48
+        return visible * 236 + (n_splats - visible) * 44 + n_splats * 68
     if stage == "offset scan":
         return n_splats * 8.0
     if stage == "emit":
@@ -48,9 +48,11 @@ def algorithmic_bytes(stage: str, n_splats: int, instances: int, visible: int, w
         # one histogram read of the keys, then per pass read + write of key and value
         return instances * (key_bytes + sort_passes * 2 * (key_bytes + 4))
     if stage == "tile ranges":
-        return instances * key_bytes + tiles * 8.0
+        # boundaries: read the keys, write the ranges; heaviest-first order: read ranges twice, write the order
+        return instances * key_bytes + tiles * 8.0 + tiles * 20.0
     if stage == "raster":
-        return instances * 44.0 + tiles * 8.0 + width * height * 12.0
+        # per instance the value and the 48 B splat record; per tile its range and order slot; RGB out
+        return instances * 52.0 + tiles * 12.0 + width * height * 12.0
     return 0.0
 
 
@@ -126,6 +128,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--forward-only", action="store_true",
                     help="time only the plain forward (for short profiler runs); the JSON line has no extra legs")
+    ap.add_argument("--fwd-bwd-only", action="store_true",
+                    help="time only forward+backward steps (for short profiler runs); prints a short JSON line")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB five times (HBM counter calibration)")
     args = ap.parse_args()
@@ -156,6 +160,27 @@ def main():
     ctx.scene_upload("bench", sc.packed(), sc.count)
     params, n = ctx.scene_params("bench")
     opts = lib.options(tile_size=16, timing=False)
+    if args.fwd_bwd_only:
+        W, H = cam.width, cam.height
+        g = torch.Generator(device="cuda").manual_seed(2)
+        dldc = torch.randn(3 * H * W, device="cuda", generator=g)
+        grad = torch.empty((59, n), dtype=torch.float32, device="cuda")
+        sopts = lib.options(tile_size=16, save_state=True, timing=True)
+        for _ in range(args.warmup):
+            ctx.forward(params, n, u, sopts, None)
+            ctx.backward(params, n, u, dldc, grad)
+        ctx.synchronize()
+        ctx.timings()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ctx.forward(params, n, u, sopts, None)
+            ctx.backward(params, n, u, dldc, grad)
+        ctx.synchronize()
+        dt = time.perf_counter() - t0
+        print(json.dumps({"metric": "fwd+bwd frames/s (profiling run)", "value": round(args.steps / dt, 3),
+                          "stage_ms": {k: round(v, 5) for k, v in ctx.timings().items()}}))
+        ctx.close()
+        return
     log(f"warmup {args.warmup} frames")
     for _ in range(args.warmup):
         ctx.forward(params, n, u, opts, None)

@@ -471,9 +471,9 @@ int launch_backward(Context* c, const float* params, uint32_t count, const CamK&
     mark(c, "backward raster");
     const uint32_t tiles = tx * ty;
     const uint32_t blocks = ceil_div(tiles * (tile * tile / 64), 4u);
-    uint32_t* order = (uint32_t*)scratch(c, "backward_tile_order", (size_t)tiles * 4);
+    // the forward's heaviest-first order of the same tile ranges
+    const uint32_t* order = (const uint32_t*)buffer(c, "frame_tile_order", 0);
     if (!order) return GSR_ERR_ALLOC;
-    if ((rc = launch_tile_order(c, ranges, tiles, order))) return rc;
     switch (tile) {
         case 8:
             hipLaunchKernelGGL(k_raster_backward<8>, dim3(blocks), dim3(256), 0, c->stream, W, H, tx, tiles, count,

@@ -228,9 +228,10 @@ int launch_sort(Context* c, uint32_t key_bytes, void* k0, uint32_t* v0, void* k1
 int launch_ranges(Context* c, const void* keys, uint32_t key_bytes, const uint32_t* n_dev,
                   uint64_t n_host, uint32_t num_tiles, uint32_t* ranges, bool clear);
 // splats bounds the per-splat records, the instance count (n_dev, else n_host) the ranges.
+// order: tiles heaviest first (launch_tile_order) or null to compute it.
 int launch_raster(Context* c, uint32_t W, uint32_t H, uint32_t tile, uint32_t splats, const uint32_t* n_dev,
                   uint32_t n_host, const uint32_t* ranges, const uint32_t* vals, const SplatRec* rec, float* out,
-                  float* tfinal, uint32_t* last);
+                  float* tfinal, uint32_t* last, const uint32_t* order = nullptr);
 int launch_fill_u32(Context* c, uint32_t* p, uint32_t value, size_t count);
 // order[0..tiles) = the tiles, heaviest (longest instance range) first.
 int launch_tile_order(Context* c, const uint2* ranges, uint32_t tiles, uint32_t* order);

@@ -67,12 +67,13 @@ int enqueue_forward(Context* c, const float* params, uint32_t count, const gsr_c
     uint32_t* v0 = (uint32_t*)buffer(c, "vals0", p.capacity * 4);
     uint32_t* v1 = (uint32_t*)buffer(c, "vals1", p.capacity * 4);
     uint32_t* ranges = (uint32_t*)buffer(c, "ranges", (size_t)p.num_tiles * 8);
+    uint32_t* order = (uint32_t*)buffer(c, "frame_tile_order", (size_t)p.num_tiles * 4);
     const size_t px = (size_t)p.W * p.H;
     float* image = image_override ? image_override : (float*)buffer(c, "image", px * 12);
     float* tfinal = save_state ? (float*)buffer(c, "tfinal", px * 4) : nullptr;
     uint32_t* last = save_state ? (uint32_t*)buffer(c, "last", px * 4) : nullptr;
     FrameCounters* counters = (FrameCounters*)buffer(c, "counters", sizeof(FrameCounters));
-    if (!tiles || !offsets || !meta || !rec || !k0 || !k1 || !v0 || !v1 || !ranges || !image ||
+    if (!tiles || !offsets || !meta || !rec || !order || !k0 || !k1 || !v0 || !v1 || !ranges || !image ||
         (save_state && (!tfinal || !last)) || !counters)
         return GSR_ERR_ALLOC;
     CamK ck;
@@ -101,9 +102,10 @@ int enqueue_forward(Context* c, const float* params, uint32_t count, const gsr_c
     void* ks = idx ? k1 : k0;
     uint32_t* vs = idx ? v1 : v0;
     if ((rc = launch_ranges(c, ks, p.key_bytes, &counters->n, 0, p.num_tiles, ranges, false))) return rc;
+    if ((rc = launch_tile_order(c, (const uint2*)ranges, p.num_tiles, order))) return rc;
     mark(c, "raster");
-    if ((rc = launch_raster(c, p.W, p.H, p.tile, count, &counters->n, 0, ranges, vs, rec, image, tfinal,
-                             last)))
+    if ((rc = launch_raster(c, p.W, p.H, p.tile, count, &counters->n, 0, ranges, vs, rec, image, tfinal, last,
+                             order)))
         return rc;
     mark_end(c);
     if (!image_override) {

@@ -372,10 +372,11 @@ static void launch_raster_tile(Context* c, uint32_t blocks, uint32_t W, uint32_t
 }
 
 // T_final and the last contributor: when only one is requested the other goes
-// to a scratch buffer (one kernel writes both or neither).
+// to a scratch buffer (one kernel writes both or neither).  order: the tiles
+// heaviest first (launch_tile_order), computed here when null.
 int launch_raster(Context* c, uint32_t W, uint32_t H, uint32_t tile, uint32_t splats, const uint32_t* n_dev,
                   uint32_t n_host, const uint32_t* ranges, const uint32_t* vals, const SplatRec* rec, float* out,
-                  float* tfinal, uint32_t* last) {
+                  float* tfinal, uint32_t* last, const uint32_t* order) {
     const uint32_t tx = ceil_div(W, tile), ty = ceil_div(H, tile);
     const uint32_t tiles = tx * ty;
     if (tiles == 0) return GSR_OK;
@@ -387,10 +388,13 @@ int launch_raster(Context* c, uint32_t W, uint32_t H, uint32_t tile, uint32_t sp
     }
     const uint2* rg = (const uint2*)ranges;
     const uint32_t blocks = ceil_div(tiles * (tile * tile / 64), 4u);
-    uint32_t* order = (uint32_t*)scratch(c, "tile_order", (size_t)tiles * 4);
-    if (!order) return GSR_ERR_ALLOC;
-    int rc = launch_tile_order(c, rg, tiles, order);
-    if (rc) return rc;
+    if (!order) {
+        uint32_t* o = (uint32_t*)scratch(c, "tile_order", (size_t)tiles * 4);
+        if (!o) return GSR_ERR_ALLOC;
+        const int rc = launch_tile_order(c, rg, tiles, o);
+        if (rc) return rc;
+        order = o;
+    }
     switch (tile) {
         case 8:
             launch_raster_tile<8>(c, blocks, W, H, tx, tiles, splats, n_dev, n_host, rg, vals, rec, out, tfinal,
