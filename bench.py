@@ -120,12 +120,57 @@ def batch_leg(ctx, torch, lib, scenes, reps=10):
             "autotune_best_ms": round(tuned_ms, 4)}
 
 
+def scale_leg(ctx, torch, lib, scenes, reps=3):
+    """6,000,000 splats, 8 cameras at 3840x2160: batched forward (saved
+    state) + backward with the gradient summed over the views, dL/dRGB ~
+    N(0, 1) resident in HBM."""
+    log("scale: building the 6M-splat scene and 8 4K cameras")
+    sc, cams = scenes.workload_6m_4k()
+    us = [c.uniform(sc.count) for c in cams]
+    n = sc.count
+    params = torch.from_numpy(sc.planar()).cuda()
+    del sc
+    opts = lib.options(tile_size=16, save_state=True)
+    offs = ctx.batch_layout(n, us, opts)
+    out = torch.empty(offs[-1], dtype=torch.float32, device="cuda")
+    g = torch.Generator(device="cuda").manual_seed(7)
+    dldc = torch.randn(offs[-1], device="cuda", generator=g)
+    grad = torch.empty((59, n), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    log("scale: warm-up (eager frame, graph capture)")
+    for _ in range(2):
+        ctx.render_batch(params, n, us, opts, out, use_graph=True)
+        ctx.backward_batch(params, n, us, dldc, grad)
+    ctx.synchronize()
+    instances = ctx.batch_instance_count()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.render_batch(params, n, us, opts, out, use_graph=True)
+        ctx.backward_batch(params, n, us, dldc, grad)
+    ctx.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    tops = lib.options(tile_size=16, save_state=True, timing=True)
+    ctx.render_batch(params, n, us, tops, out, use_graph=False)
+    ctx.backward_batch(params, n, us, dldc, grad)
+    ctx.synchronize()
+    stage = {k: round(v, 4) for k, v in ctx.timings().items()}
+    finite = bool(torch.isfinite(grad).all().item())
+    del params, out, dldc, grad
+    torch.cuda.empty_cache()
+    return {"workload": "batched fwd+bwd, 6,000,000 splats x 8 views at 3840x2160, gradients summed over views",
+            "splats": n, "views": len(us), "instances": instances, "ms_per_step": round(dt * 1e3, 3),
+            "steps_per_s": round(1.0 / dt, 3), "views_per_s": round(len(us) / dt, 2), "stage_ms": stage,
+            "grad_finite": finite}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-scale", action="store_true", help="skip the 6M-splat 8 x 4K fwd+bwd batch leg")
+    ap.add_argument("--scale-only", action="store_true", help="run only the 6M-splat 8 x 4K fwd+bwd batch leg")
     ap.add_argument("--forward-only", action="store_true",
                     help="time only the plain forward (for short profiler runs); the JSON line has no extra legs")
     ap.add_argument("--fwd-bwd-only", action="store_true",
@@ -152,6 +197,12 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes": src.numel() * 4, "copies": 5}))
+        return
+    if args.scale_only:
+        ctx = lib.Context(0)
+        print(json.dumps({"metric": "batched fwd+bwd steps/s (scale leg only)",
+                          **scale_leg(ctx, torch, lib, scenes, reps=max(1, args.steps))}))
+        ctx.close()
         return
     log("building the 1M-splat scene")
     sc, cam = scenes.workload_1m_1080p()
@@ -270,6 +321,8 @@ def main():
     del dldc, grad
     ctx.scene_free("bench")
     extra["batch"] = batch_leg(ctx, torch, lib, scenes)
+    if not args.no_scale:
+        extra["scale_6m_8x4k"] = scale_leg(ctx, torch, lib, scenes)
     cpu = None
     if not args.no_cpu:
         log("cpu baseline")

@@ -206,7 +206,8 @@ int gsr_backward(gsr_context* ctx, const float* params_dev, uint32_t count,
  * unchanged; cameras may change between replays.  When tile instances
  * overflow the capacity (opts->instance_capacity, else a planner that starts
  * at 16 per splat-view and grows) the frame is re-run once with room for the
- * measured count.  Blocks until the frame is done.
+ * measured count.  Blocks until the frame is done.  opts->save_state = 1
+ * keeps every view's per-pixel state for gsr_backward_batch.
  */
 int gsr_render_batch(gsr_context* ctx, const float* params_dev, uint32_t count,
                      const gsr_camera* cams, uint32_t views, const gsr_options* opts,
@@ -217,7 +218,7 @@ int gsr_batch_instance_count(gsr_context* ctx, uint64_t* instances);
 
 /*
  * Autotuner for a batch shape: times tile size {8, 16, 32} x persistent raster
- * waves per CU {8, 16, 32} x onesweep blocks per CU {2, 3} (three timed
+ * waves per CU {8, 16, 32} x onesweep blocks per CU {1, 2, 4} (three timed
  * frames each, HIP events) and writes the fastest into *inout (other fields
  * kept).  out_rgb_dev is used as the scratch output.  The tile size changes
  * which splats reach a pixel's tile at the fringe of their 3-sigma square, so
@@ -226,6 +227,19 @@ int gsr_batch_instance_count(gsr_context* ctx, uint64_t* instances);
 int gsr_autotune_batch(gsr_context* ctx, const float* params_dev, uint32_t count,
                        const gsr_camera* cams, uint32_t views, gsr_options* inout,
                        float* out_rgb_dev, double* best_ms);
+
+/*
+ * Gradients of the last gsr_render_batch (which must have had save_state = 1
+ * and the same params, count and cameras), summed over its views.
+ * dL_drgb_dev: in the batch output layout (out_offsets).  grad_params_dev:
+ * f32[59][N], overwritten.  grad2d_dev (may be NULL): the 2-D gradient of
+ * every (view, splat), f32[views][N][12] in gsr_backward's layout.  Enqueues
+ * on the context stream and returns without waiting.  Gradients accumulate
+ * with float atomics, so the last bits may differ between runs.
+ */
+int gsr_backward_batch(gsr_context* ctx, const float* params_dev, uint32_t count,
+                       const gsr_camera* cams, uint32_t views, const float* dL_drgb_dev,
+                       float* grad_params_dev, float* grad2d_dev);
 
 #ifdef __cplusplus
 }

@@ -108,22 +108,17 @@ struct SplatB {
 // waves take the tiles heaviest first (order[], as in the forward).  Each
 // wave stages its batch of 64 splats in its own LDS slots and reads them back
 // with same-address loads.
-template <int TS>
-__global__ __launch_bounds__(256) void k_raster_backward(
-    uint32_t W, uint32_t H, uint32_t tiles_x, uint32_t num_tiles, uint32_t splats, const uint32_t* n_dev,
-    const uint2* __restrict__ ranges, const uint32_t* __restrict__ vals, const SplatRec* __restrict__ rec,
-    const float* __restrict__ tfinal,
-    const uint32_t* __restrict__ last, const float* __restrict__ dLdC, float* __restrict__ grad2d,
-    const uint32_t* __restrict__ order) {
-    constexpr uint32_t SIDE = TS / 8, BLOCKS = SIDE * SIDE;
+// One 8x8 block (pixels x0.. x0 + 7, y0.. y0 + 7 of an image of W x H whose
+// planes start at tfinal / last / dLdC) replaying the tile's instances
+// [start, min(range_end, start + max last contributor)) back to front.
+// grad2d receives, per entry of vals, the moment sums of the kernel comment.
+__device__ __forceinline__ void backward_block(uint32_t W, uint32_t H, uint32_t x0, uint32_t y0, uint32_t start,
+                                               uint32_t range_end, uint32_t splats, const uint32_t* __restrict__ vals,
+                                               const SplatRec* __restrict__ rec, const float* __restrict__ tfinal,
+                                               const uint32_t* __restrict__ last, const float* __restrict__ dLdC,
+                                               float* __restrict__ grad2d) {
     __shared__ SplatB s_spl[4][64];
-    const uint32_t wv = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const uint32_t slot = wv / BLOCKS, sub = wv % BLOCKS;
-    if (slot >= num_tiles) return;
-    const uint32_t tile = order[slot];
     const uint32_t lane = threadIdx.x & 63;
-    const uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
-    const uint32_t x0 = tx * TS + (sub % SIDE) * 8, y0 = ty * TS + (sub / SIDE) * 8;
     const uint32_t x = x0 + lane % 8, y = y0 + lane / 8;
     const float fx = (float)x, fy = (float)y;
     const size_t plane = (size_t)W * H;
@@ -139,10 +134,7 @@ __global__ __launch_bounds__(256) void k_raster_backward(
     }
     const uint32_t maxlast = wave_max_u32(lst);
     if (maxlast == 0) return;
-    const uint2 r = ranges[tile];
-    const uint32_t n_inst = *n_dev;
-    const uint32_t start = r.x;
-    const uint32_t stop = min(min(r.y, n_inst), start + maxlast);  // exclusive end of the replay
+    const uint32_t stop = min(range_end, start + maxlast);  // exclusive end of the replay
     if (stop <= start) return;
     const uint32_t tag = fold_tags();
     const float rxa = (float)x0, rxb = (float)min(x0 + 7, W - 1);
@@ -224,6 +216,53 @@ __global__ __launch_bounds__(256) void k_raster_backward(
     }
 }
 
+template <int TS>
+__global__ __launch_bounds__(256) void k_raster_backward(
+    uint32_t W, uint32_t H, uint32_t tiles_x, uint32_t num_tiles, uint32_t splats, const uint32_t* n_dev,
+    const uint2* __restrict__ ranges, const uint32_t* __restrict__ vals, const SplatRec* __restrict__ rec,
+    const float* __restrict__ tfinal, const uint32_t* __restrict__ last, const float* __restrict__ dLdC,
+    float* __restrict__ grad2d, const uint32_t* __restrict__ order) {
+    constexpr uint32_t SIDE = TS / 8, BLOCKS = SIDE * SIDE;
+    const uint32_t wv = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t slot = wv / BLOCKS, sub = wv % BLOCKS;
+    if (slot >= num_tiles) return;
+    const uint32_t tile = order[slot];
+    const uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
+    const uint2 r = ranges[tile];
+    backward_block(W, H, tx * TS + (sub % SIDE) * 8, ty * TS + (sub / SIDE) * 8, r.x, min(r.y, *n_dev), splats, vals,
+                   rec, tfinal, last, dLdC, grad2d);
+}
+
+// View batches: the tiles of every view, heaviest first (order[] over the
+// global tile ids); views[v] = {W, H, tiles_x, tile_base}, pix[v] the float
+// offset of view v's [3][H][W] image (and pix[v] / 3 its pixel offset).
+// Entries (vals) are v * n + g.
+template <int TS>
+__global__ __launch_bounds__(256) void k_raster_backward_batch(
+    const uint4* __restrict__ views, const uint64_t* __restrict__ pix, uint32_t num_views, uint32_t total_tiles,
+    uint32_t entries, const uint32_t* n_dev, const uint2* __restrict__ ranges, const uint32_t* __restrict__ vals,
+    const SplatRec* __restrict__ rec, const float* __restrict__ tfinal, const uint32_t* __restrict__ last,
+    const float* __restrict__ dLdC, float* __restrict__ grad2d, const uint32_t* __restrict__ order) {
+    constexpr uint32_t SIDE = TS / 8, BLOCKS = SIDE * SIDE;
+    const uint32_t wv = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t slot = wv / BLOCKS, sub = wv % BLOCKS;
+    if (slot >= total_tiles) return;
+    const uint32_t t = order[slot];
+    uint32_t lo = 0, hi = num_views;  // view holding global tile t: last v with tile_base <= t
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (views[mid].w <= t) lo = mid;
+        else hi = mid;
+    }
+    const uint4 vi = views[lo];
+    const uint32_t local = t - vi.w;
+    const uint32_t tx = local % vi.z, ty = local / vi.z;
+    const uint2 r = ranges[t];
+    const uint64_t po = pix[lo] / 3;
+    backward_block(vi.x, vi.y, tx * TS + (sub % SIDE) * 8, ty * TS + (sub / SIDE) * 8, r.x, min(r.y, *n_dev), entries,
+                   vals, rec, tfinal + po, last + po, dLdC + pix[lo], grad2d);
+}
+
 constexpr float SH_C0 = 0.28209479177387814f;
 constexpr float SH_C1 = 0.4886025119029199f;
 constexpr float SH_C2_0 = 1.0925484305920792f, SH_C2_1 = -1.0925484305920792f,
@@ -234,46 +273,77 @@ constexpr float SH_C3_0 = -0.5900435899266435f, SH_C3_1 = 2.890611442640554f,
                 SH_C3_4 = -0.4570457994644658f, SH_C3_5 = 1.445305721320277f,
                 SH_C3_6 = -0.5900435899266435f;
 
-// grad2d arrives with the raster's per-splat sums (moments of dL/dpower in
-// slots 0-4) and leaves in the 2-D gradient layout: d/dx = -(A Mx + B My),
+// The raster's per-splat sums (moments of dL/dpower in slots 0-4, then
+// opacity and colour) in the 2-D gradient layout: d/dx = -(A Mx + B My),
 // d/dy = -(C My + B Mx), d/dA = -Mxx / 2, d/dB = -Mxy, d/dC = -Myy / 2, with
 // the splat's forward conic (A, B, C).
-__global__ __launch_bounds__(256) void k_preprocess_backward(const float* __restrict__ P, uint32_t n, CamK cam,
-                                                             const SplatRec* __restrict__ rec,
-                                                             float4* __restrict__ grad2d,
-                                                             float* __restrict__ grad) {
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= n) return;
-    const size_t N = n;
-    const float4 G0 = grad2d[(size_t)g * 3], G1 = grad2d[(size_t)g * 3 + 1], G2 = grad2d[(size_t)g * 3 + 2];
-    bool any = false;
-    {
-        const float m[9] = {G0.x, G0.y, G0.z, G0.w, G1.x, G1.y, G1.z, G1.w, G2.x};
+__device__ __forceinline__ void moments_to_2d(float4 G0, float4 G1, float4 G2, float4 cn, float (&Gv)[9]) {
+    Gv[0] = -__builtin_fmaf(cn.x, G0.x, cn.y * G0.y);
+    Gv[1] = -__builtin_fmaf(cn.z, G0.y, cn.y * G0.x);
+    Gv[2] = -0.5f * G0.z;
+    Gv[3] = -G0.w;
+    Gv[4] = -0.5f * G1.x;
+    Gv[5] = G1.y;
+    Gv[6] = G1.z;
+    Gv[7] = G1.w;
+    Gv[8] = G2.x;
+}
+
+// View-independent part of one splat for the preprocess adjoint (the
+// forward's covariance construction, preprocess.wgsl order).
+struct SplatShape {
+    float px, py, pz;
+    float sv[3];                    // exp(log-scale)
+    float qw, qx, qy, qz, ql;       // normalised rotation, length of the raw one
+    float R[3][3], M[3][3], S[3][3];  // rotation, R diag(s), 3-D covariance
+};
+
+__device__ __forceinline__ void splat_shape(const float* __restrict__ P, size_t N, uint32_t g, SplatShape& s) {
+    s.px = P[CH_POS * N + g];
+    s.py = P[(CH_POS + 1) * N + g];
+    s.pz = P[(CH_POS + 2) * N + g];
 #pragma unroll
-        for (int k = 0; k < 9; ++k) any |= m[k] != 0.0f;
-    }
-    float Gv[9] = {0.f, 0.f, 0.f, 0.f, 0.f, G1.y, G1.z, G1.w, G2.x};
-    if (any) {
-        const float4 cn = rec[g].co;
-        Gv[0] = -__builtin_fmaf(cn.x, G0.x, cn.y * G0.y);
-        Gv[1] = -__builtin_fmaf(cn.z, G0.y, cn.y * G0.x);
-        Gv[2] = -0.5f * G0.z;
-        Gv[3] = -G0.w;
-        Gv[4] = -0.5f * G1.x;
-        grad2d[(size_t)g * 3] = make_float4(Gv[0], Gv[1], Gv[2], Gv[3]);
-        grad2d[(size_t)g * 3 + 1] = make_float4(Gv[4], G1.y, G1.z, G1.w);
-    }
-    const float px = P[CH_POS * N + g], py = P[(CH_POS + 1) * N + g], pz = P[(CH_POS + 2) * N + g];
+    for (int c = 0; c < 3; ++c) s.sv[c] = gsr_exp(P[(CH_SCALE + c) * N + g]);
+    const float rw = P[(CH_ROT + 0) * N + g], rx = P[(CH_ROT + 1) * N + g];
+    const float ry = P[(CH_ROT + 2) * N + g], rz = P[(CH_ROT + 3) * N + g];
+    s.ql = sqrtf(((rx * rx + ry * ry) + rz * rz) + rw * rw);
+    const float qw = rw / s.ql, qx = rx / s.ql, qy = ry / s.ql, qz = rz / s.ql;
+    s.qw = qw;
+    s.qx = qx;
+    s.qy = qy;
+    s.qz = qz;
+    s.R[0][0] = 1.0f - 2.0f * (qy * qy + qz * qz); s.R[0][1] = 2.0f * (qx * qy - qw * qz); s.R[0][2] = 2.0f * (qx * qz + qw * qy);
+    s.R[1][0] = 2.0f * (qx * qy + qw * qz); s.R[1][1] = 1.0f - 2.0f * (qx * qx + qz * qz); s.R[1][2] = 2.0f * (qy * qz - qw * qx);
+    s.R[2][0] = 2.0f * (qx * qz - qw * qy); s.R[2][1] = 2.0f * (qy * qz + qw * qx); s.R[2][2] = 1.0f - 2.0f * (qx * qx + qy * qy);
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) s.M[r][c] = s.R[r][c] * s.sv[c];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            s.S[i][j] = s.M[i][0] * s.M[j][0] + s.M[i][1] * s.M[j][1] + s.M[i][2] * s.M[j][2];
+}
+
+// Adjoint of one view's projection of the splat, for the 2-D gradient Gv
+// (pixel x, y; conic A, B, C; opacity; r, g, b).  Returns false when the
+// splat is behind the near plane of this view (no gradient).  Otherwise:
+//   gp    dL/dposition through the pixel position and the covariance;
+//   gsym  dL/dS + (dL/dS)^T of the 3-D covariance;
+//   gch   dL/d(SH colour) per channel (zero where the forward clamped it);
+//   basis the 15 higher SH basis values of the view direction;
+// and the view direction's part of dL/dposition is added to gp.
+__device__ __forceinline__ bool view_adjoint(const float* __restrict__ P, size_t N, uint32_t g, const SplatShape& s,
+                                             const CamK& cam, const float (&Gv)[9], float (&gp)[3],
+                                             float (&gsym)[3][3], float (&gch)[3], float (&basis)[15]) {
+    const float px = s.px, py = s.py, pz = s.pz;
     const float* V = cam.view;
     const float* PV = cam.view_proj;
     const float tx = ((V[0] * px + V[4] * py) + V[8] * pz) + V[12];
     const float ty = ((V[1] * px + V[5] * py) + V[9] * pz) + V[13];
     const float tz = ((V[2] * px + V[6] * py) + V[10] * pz) + V[14];
-    if (!any || !(tz < -kNearView)) {
-        for (int ch = 0; ch < (int)kParams; ++ch) grad[(size_t)ch * N + g] = 0.0f;
-        return;
-    }
-    float gp0, gp1, gp2;
+    if (!(tz < -kNearView)) return false;
     {   // pixel position <- ndc <- clip <- p
         const float cx = ((PV[0] * px + PV[4] * py) + PV[8] * pz) + PV[12];
         const float cy = ((PV[1] * px + PV[5] * py) + PV[9] * pz) + PV[13];
@@ -282,32 +352,11 @@ __global__ __launch_bounds__(256) void k_preprocess_backward(const float* __rest
         const float gnx = Gv[0] * cam.texture_size[0] * 0.5f, gny = Gv[1] * cam.texture_size[1] * 0.5f;
         const float iw = 1.0f / w;
         const float gcx = gnx * iw, gcy = gny * iw, gcw = -(gnx * cx + gny * cy) * iw * iw;
-        gp0 = PV[0] * gcx + PV[1] * gcy + PV[3] * gcw;
-        gp1 = PV[4] * gcx + PV[5] * gcy + PV[7] * gcw;
-        gp2 = PV[8] * gcx + PV[9] * gcy + PV[11] * gcw;
+        gp[0] = PV[0] * gcx + PV[1] * gcy + PV[3] * gcw;
+        gp[1] = PV[4] * gcx + PV[5] * gcy + PV[7] * gcw;
+        gp[2] = PV[8] * gcx + PV[9] * gcy + PV[11] * gcw;
     }
     // covariance chain
-    const float s0 = gsr_exp(P[(CH_SCALE + 0) * N + g]);
-    const float s1 = gsr_exp(P[(CH_SCALE + 1) * N + g]);
-    const float s2 = gsr_exp(P[(CH_SCALE + 2) * N + g]);
-    const float rw = P[(CH_ROT + 0) * N + g], rx = P[(CH_ROT + 1) * N + g];
-    const float ry = P[(CH_ROT + 2) * N + g], rz = P[(CH_ROT + 3) * N + g];
-    const float ql = sqrtf(((rx * rx + ry * ry) + rz * rz) + rw * rw);
-    const float qw = rw / ql, qx = rx / ql, qy = ry / ql, qz = rz / ql;
-    float R[3][3];
-    R[0][0] = 1.0f - 2.0f * (qy * qy + qz * qz); R[0][1] = 2.0f * (qx * qy - qw * qz); R[0][2] = 2.0f * (qx * qz + qw * qy);
-    R[1][0] = 2.0f * (qx * qy + qw * qz); R[1][1] = 1.0f - 2.0f * (qx * qx + qz * qz); R[1][2] = 2.0f * (qy * qz - qw * qx);
-    R[2][0] = 2.0f * (qx * qz - qw * qy); R[2][1] = 2.0f * (qy * qz + qw * qx); R[2][2] = 1.0f - 2.0f * (qx * qx + qy * qy);
-    const float sv[3] = {s0, s1, s2};
-    float M[3][3], S[3][3];
-#pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) M[r][c] = R[r][c] * sv[c];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) S[i][j] = M[i][0] * M[j][0] + M[i][1] * M[j][1] + M[i][2] * M[j][2];
     const float limx = kFrustumGuard * cam.tan_fov[0], limy = kFrustumGuard * cam.tan_fov[1];
     const float vx = tx / tz, vy = ty / tz;
     const bool clx = vx < -limx || vx > limx, cly = vy < -limy || vy > limy;
@@ -326,8 +375,8 @@ __global__ __launch_bounds__(256) void k_preprocess_backward(const float* __rest
     float St0[3], St1[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-        St0[i] = S[i][0] * Tm[0][0] + S[i][1] * Tm[0][1] + S[i][2] * Tm[0][2];
-        St1[i] = S[i][0] * Tm[1][0] + S[i][1] * Tm[1][1] + S[i][2] * Tm[1][2];
+        St0[i] = s.S[i][0] * Tm[0][0] + s.S[i][1] * Tm[0][1] + s.S[i][2] * Tm[0][2];
+        St1[i] = s.S[i][0] * Tm[1][0] + s.S[i][1] * Tm[1][1] + s.S[i][2] * Tm[1][2];
     }
     const float a = Tm[0][0] * St0[0] + Tm[0][1] * St0[1] + Tm[0][2] * St0[2] + 0.3f;
     const float b = Tm[0][0] * St1[0] + Tm[0][1] * St1[1] + Tm[0][2] * St1[2];
@@ -349,6 +398,10 @@ __global__ __launch_bounds__(256) void k_preprocess_backward(const float* __rest
 #pragma unroll
         for (int j = 0; j < 3; ++j)
             gS[i][j] = ga * Tm[0][i] * Tm[0][j] + gbb * Tm[0][i] * Tm[1][j] + gc * Tm[1][i] * Tm[1][j];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) gsym[i][j] = gS[i][j] + gS[j][i];
     float gJ00 = 0.f, gJ02 = 0.f, gJ11 = 0.f, gJ12 = 0.f;
 #pragma unroll
     for (int cc = 0; cc < 3; ++cc) {
@@ -363,53 +416,19 @@ __global__ __launch_bounds__(256) void k_preprocess_backward(const float* __rest
     float gtx = clx ? 0.0f : gux, gty = cly ? 0.0f : guy, gtz = guz;
     if (clx) gtz += gux * (vx > 0.0f ? limx : -limx);
     if (cly) gtz += guy * (vy > 0.0f ? limy : -limy);
-    gp0 += V[0] * gtx + V[1] * gty + V[2] * gtz;
-    gp1 += V[4] * gtx + V[5] * gty + V[6] * gtz;
-    gp2 += V[8] * gtx + V[9] * gty + V[10] * gtz;
-    float gM[3][3];
-#pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-        for (int cc = 0; cc < 3; ++cc)
-            gM[r][cc] = (gS[r][0] + gS[0][r]) * M[0][cc] + (gS[r][1] + gS[1][r]) * M[1][cc] +
-                        (gS[r][2] + gS[2][r]) * M[2][cc];
-    float gR[3][3];
-#pragma unroll
-    for (int cc = 0; cc < 3; ++cc) {
-        float gsc = 0.f;
-#pragma unroll
-        for (int r = 0; r < 3; ++r) {
-            gR[r][cc] = gM[r][cc] * sv[cc];
-            gsc += gM[r][cc] * R[r][cc];
-        }
-        grad[(size_t)(CH_SCALE + cc) * N + g] = gsc * sv[cc];
-    }
-    const float gqw = 2.0f * (-qz * gR[0][1] + qy * gR[0][2] + qz * gR[1][0] - qx * gR[1][2] - qy * gR[2][0] + qx * gR[2][1]);
-    const float gqx = 2.0f * (qy * gR[0][1] + qz * gR[0][2] + qy * gR[1][0] - 2.0f * qx * gR[1][1] - qw * gR[1][2] +
-                              qz * gR[2][0] + qw * gR[2][1] - 2.0f * qx * gR[2][2]);
-    const float gqy = 2.0f * (-2.0f * qy * gR[0][0] + qx * gR[0][1] + qw * gR[0][2] + qx * gR[1][0] + qz * gR[1][2] -
-                              qw * gR[2][0] + qz * gR[2][1] - 2.0f * qy * gR[2][2]);
-    const float gqz = 2.0f * (-2.0f * qz * gR[0][0] - qw * gR[0][1] + qx * gR[0][2] + qw * gR[1][0] - 2.0f * qz * gR[1][1] +
-                              qy * gR[1][2] + qx * gR[2][0] + qy * gR[2][1]);
-    const float dotq = qw * gqw + qx * gqx + qy * gqy + qz * gqz;
-    const float iql = 1.0f / ql;
-    grad[(size_t)(CH_ROT + 0) * N + g] = (gqw - qw * dotq) * iql;
-    grad[(size_t)(CH_ROT + 1) * N + g] = (gqx - qx * dotq) * iql;
-    grad[(size_t)(CH_ROT + 2) * N + g] = (gqy - qy * dotq) * iql;
-    grad[(size_t)(CH_ROT + 3) * N + g] = (gqz - qz * dotq) * iql;
-    // opacity
-    const float op = 1.0f / (1.0f + gsr_exp(-P[CH_OPACITY * N + g]));
-    grad[(size_t)CH_OPACITY * N + g] = Gv[5] * op * (1.0f - op);
+    gp[0] += V[0] * gtx + V[1] * gty + V[2] * gtz;
+    gp[1] += V[4] * gtx + V[5] * gty + V[6] * gtz;
+    gp[2] += V[8] * gtx + V[9] * gty + V[10] * gtz;
     // colour
     const float vx3 = px - cam.eye[0], vy3 = py - cam.eye[1], vz3 = pz - cam.eye[2];
     const float vl = sqrtf(vx3 * vx3 + vy3 * vy3 + vz3 * vz3);
     const float X = vx3 / vl, Y = vy3 / vl, Z = vz3 / vl;
     const float XX = X * X, YY = Y * Y, ZZ = Z * Z;
-    const float basis[15] = {-SH_C1 * Y, SH_C1 * Z, -SH_C1 * X, SH_C2_0 * X * Y, SH_C2_1 * Y * Z,
-                             SH_C2_2 * (2.0f * ZZ - XX - YY), SH_C2_3 * X * Z, SH_C2_4 * (XX - YY),
-                             SH_C3_0 * Y * (3.0f * XX - YY), SH_C3_1 * X * Y * Z, SH_C3_2 * Y * (4.0f * ZZ - XX - YY),
-                             SH_C3_3 * Z * (2.0f * ZZ - 3.0f * XX - 3.0f * YY), SH_C3_4 * X * (4.0f * ZZ - XX - YY),
-                             SH_C3_5 * Z * (XX - YY), SH_C3_6 * X * (XX - 3.0f * YY)};
+    const float bs[15] = {-SH_C1 * Y, SH_C1 * Z, -SH_C1 * X, SH_C2_0 * X * Y, SH_C2_1 * Y * Z,
+                          SH_C2_2 * (2.0f * ZZ - XX - YY), SH_C2_3 * X * Z, SH_C2_4 * (XX - YY),
+                          SH_C3_0 * Y * (3.0f * XX - YY), SH_C3_1 * X * Y * Z, SH_C3_2 * Y * (4.0f * ZZ - XX - YY),
+                          SH_C3_3 * Z * (2.0f * ZZ - 3.0f * XX - 3.0f * YY), SH_C3_4 * X * (4.0f * ZZ - XX - YY),
+                          SH_C3_5 * Z * (XX - YY), SH_C3_6 * X * (XX - 3.0f * YY)};
     const float dbx[15] = {0.f, 0.f, -SH_C1, SH_C2_0 * Y, 0.f, SH_C2_2 * (-2.0f * X), SH_C2_3 * Z, SH_C2_4 * 2.0f * X,
                            SH_C3_0 * Y * 6.0f * X, SH_C3_1 * Y * Z, SH_C3_2 * Y * (-2.0f * X), SH_C3_3 * Z * (-6.0f * X),
                            SH_C3_4 * (4.0f * ZZ - 3.0f * XX - YY), SH_C3_5 * Z * 2.0f * X, SH_C3_6 * (3.0f * XX - 3.0f * YY)};
@@ -420,6 +439,8 @@ __global__ __launch_bounds__(256) void k_preprocess_backward(const float* __rest
     const float dbz[15] = {0.f, SH_C1, 0.f, 0.f, SH_C2_1 * Y, SH_C2_2 * 4.0f * Z, SH_C2_3 * X, 0.f, 0.f,
                            SH_C3_1 * X * Y, SH_C3_2 * Y * 8.0f * Z, SH_C3_3 * (6.0f * ZZ - 3.0f * XX - 3.0f * YY),
                            SH_C3_4 * X * 8.0f * Z, SH_C3_5 * (XX - YY), 0.f};
+#pragma unroll
+    for (int k = 0; k < 15; ++k) basis[k] = bs[k];
     float gdx = 0.f, gdy = 0.f, gdz = 0.f;
     const float gcol[3] = {Gv[6], Gv[7], Gv[8]};
 #pragma unroll
@@ -429,26 +450,178 @@ __global__ __launch_bounds__(256) void k_preprocess_backward(const float* __rest
 #pragma unroll
         for (int k = 0; k < 15; ++k) {
             f[k] = P[(CH_REST + 15 * ch + k) * N + g];
-            acc += basis[k] * f[k];
+            acc += bs[k] * f[k];
         }
-        const float gch = (acc + 0.5f < 0.0f) ? 0.0f : gcol[ch];
-        grad[(size_t)(CH_DC + ch) * N + g] = SH_C0 * gch;
+        const float gv = (acc + 0.5f < 0.0f) ? 0.0f : gcol[ch];
+        gch[ch] = gv;
 #pragma unroll
         for (int k = 0; k < 15; ++k) {
-            grad[(size_t)(CH_REST + 15 * ch + k) * N + g] = basis[k] * gch;
-            gdx += gch * dbx[k] * f[k];
-            gdy += gch * dby[k] * f[k];
-            gdz += gch * dbz[k] * f[k];
+            gdx += gv * dbx[k] * f[k];
+            gdy += gv * dby[k] * f[k];
+            gdz += gv * dbz[k] * f[k];
         }
     }
     const float dd = X * gdx + Y * gdy + Z * gdz;
     const float ivl = 1.0f / vl;
-    gp0 += (gdx - X * dd) * ivl;
-    gp1 += (gdy - Y * dd) * ivl;
-    gp2 += (gdz - Z * dd) * ivl;
-    grad[(size_t)(CH_POS + 0) * N + g] = gp0;
-    grad[(size_t)(CH_POS + 1) * N + g] = gp1;
-    grad[(size_t)(CH_POS + 2) * N + g] = gp2;
+    gp[0] += (gdx - X * dd) * ivl;
+    gp[1] += (gdy - Y * dd) * ivl;
+    gp[2] += (gdz - Z * dd) * ivl;
+    return true;
+}
+
+// dL/dS + (dL/dS)^T -> gs: dL/d log-scale (3), dL/d raw quaternion (4).
+__device__ __forceinline__ void shape_adjoint(const SplatShape& s, const float (&gsym)[3][3], float (&gs)[7]) {
+    float gM[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc)
+            gM[r][cc] = gsym[r][0] * s.M[0][cc] + gsym[r][1] * s.M[1][cc] + gsym[r][2] * s.M[2][cc];
+    float gR[3][3];
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc) {
+        float gsc = 0.f;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            gR[r][cc] = gM[r][cc] * s.sv[cc];
+            gsc += gM[r][cc] * s.R[r][cc];
+        }
+        gs[cc] = gsc * s.sv[cc];
+    }
+    const float qw = s.qw, qx = s.qx, qy = s.qy, qz = s.qz;
+    const float gqw = 2.0f * (-qz * gR[0][1] + qy * gR[0][2] + qz * gR[1][0] - qx * gR[1][2] - qy * gR[2][0] + qx * gR[2][1]);
+    const float gqx = 2.0f * (qy * gR[0][1] + qz * gR[0][2] + qy * gR[1][0] - 2.0f * qx * gR[1][1] - qw * gR[1][2] +
+                              qz * gR[2][0] + qw * gR[2][1] - 2.0f * qx * gR[2][2]);
+    const float gqy = 2.0f * (-2.0f * qy * gR[0][0] + qx * gR[0][1] + qw * gR[0][2] + qx * gR[1][0] + qz * gR[1][2] -
+                              qw * gR[2][0] + qz * gR[2][1] - 2.0f * qy * gR[2][2]);
+    const float gqz = 2.0f * (-2.0f * qz * gR[0][0] - qw * gR[0][1] + qx * gR[0][2] + qw * gR[1][0] - 2.0f * qz * gR[1][1] +
+                              qy * gR[1][2] + qx * gR[2][0] + qy * gR[2][1]);
+    const float dotq = qw * gqw + qx * gqx + qy * gqy + qz * gqz;
+    const float iql = 1.0f / s.ql;
+    gs[3] = (gqw - qw * dotq) * iql;
+    gs[4] = (gqx - qx * dotq) * iql;
+    gs[5] = (gqy - qy * dotq) * iql;
+    gs[6] = (gqz - qz * dotq) * iql;
+}
+
+__device__ __forceinline__ float opacity_of(const float* __restrict__ P, size_t N, uint32_t g) {
+    return 1.0f / (1.0f + gsr_exp(-P[CH_OPACITY * N + g]));
+}
+
+// grad2d arrives with the raster's per-splat sums and leaves in the 2-D
+// gradient layout (moments_to_2d); grad receives the 59 parameter gradients.
+__global__ __launch_bounds__(256) void k_preprocess_backward(const float* __restrict__ P, uint32_t n, CamK cam,
+                                                             const SplatRec* __restrict__ rec,
+                                                             float4* __restrict__ grad2d,
+                                                             float* __restrict__ grad) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const size_t N = n;
+    const float4 G0 = grad2d[(size_t)g * 3], G1 = grad2d[(size_t)g * 3 + 1], G2 = grad2d[(size_t)g * 3 + 2];
+    const bool any = (G0.x != 0.0f) | (G0.y != 0.0f) | (G0.z != 0.0f) | (G0.w != 0.0f) | (G1.x != 0.0f) |
+                     (G1.y != 0.0f) | (G1.z != 0.0f) | (G1.w != 0.0f) | (G2.x != 0.0f);
+    float Gv[9];
+    if (any) {
+        moments_to_2d(G0, G1, G2, rec[g].co, Gv);
+        grad2d[(size_t)g * 3] = make_float4(Gv[0], Gv[1], Gv[2], Gv[3]);
+        grad2d[(size_t)g * 3 + 1] = make_float4(Gv[4], G1.y, G1.z, G1.w);
+    }
+    SplatShape s;
+    float gp[3], gsym[3][3], gch[3], basis[15];
+    bool ok = any;
+    if (ok) {
+        splat_shape(P, N, g, s);
+        ok = view_adjoint(P, N, g, s, cam, Gv, gp, gsym, gch, basis);
+    }
+    if (!ok) {
+        for (int ch = 0; ch < (int)kParams; ++ch) grad[(size_t)ch * N + g] = 0.0f;
+        return;
+    }
+    float gs[7];
+    shape_adjoint(s, gsym, gs);
+#pragma unroll
+    for (int k = 0; k < 7; ++k) grad[(size_t)(CH_SCALE + k) * N + g] = gs[k];  // scale, then rotation
+    const float op = opacity_of(P, N, g);
+    grad[(size_t)CH_OPACITY * N + g] = Gv[5] * op * (1.0f - op);
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+        grad[(size_t)(CH_DC + ch) * N + g] = SH_C0 * gch[ch];
+#pragma unroll
+        for (int k = 0; k < 15; ++k) grad[(size_t)(CH_REST + 15 * ch + k) * N + g] = basis[k] * gch[ch];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) grad[(size_t)(CH_POS + c) * N + g] = gp[c];
+}
+
+// View batches: one thread per splat walks the views, converting each
+// entry's sums (entry v * n + g of grad2d, conic from rec at the same index)
+// and summing the parameter gradient over views: each view's gradient is
+// formed as in k_preprocess_backward and added; the opacity and colour ones
+// are linear in the view's 2-D gradient and are summed before the last
+// (view-independent) factor.  grad2d_out (nullable) receives every entry in
+// the 2-D gradient layout.
+__global__ __launch_bounds__(256) void k_preprocess_backward_batch(const float* __restrict__ P, uint32_t n,
+                                                                   const CamK* __restrict__ cams, uint32_t views,
+                                                                   const SplatRec* __restrict__ rec,
+                                                                   const float4* __restrict__ grad2d,
+                                                                   float4* __restrict__ grad2d_out,
+                                                                   float* __restrict__ grad) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const size_t N = n;
+    SplatShape s;
+    bool shaped = false, touched = false;
+    float sp[3] = {0.f, 0.f, 0.f}, sgs[7] = {}, sop = 0.0f, sch[3] = {0.f, 0.f, 0.f}, srest[45] = {};
+    for (uint32_t v = 0; v < views; ++v) {
+        const size_t e = (size_t)v * N + g;
+        const float4 G0 = grad2d[e * 3], G1 = grad2d[e * 3 + 1], G2 = grad2d[e * 3 + 2];
+        const bool any = (G0.x != 0.0f) | (G0.y != 0.0f) | (G0.z != 0.0f) | (G0.w != 0.0f) | (G1.x != 0.0f) |
+                         (G1.y != 0.0f) | (G1.z != 0.0f) | (G1.w != 0.0f) | (G2.x != 0.0f);
+        float Gv[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        if (any) moments_to_2d(G0, G1, G2, rec[e].co, Gv);
+        if (grad2d_out) {
+            grad2d_out[e * 3] = make_float4(Gv[0], Gv[1], Gv[2], Gv[3]);
+            grad2d_out[e * 3 + 1] = make_float4(Gv[4], Gv[5], Gv[6], Gv[7]);
+            grad2d_out[e * 3 + 2] = make_float4(Gv[8], 0.0f, 0.0f, 0.0f);
+        }
+        float gp[3], gsym[3][3], gch[3], basis[15];
+        if (!any) continue;
+        if (!shaped) {
+            splat_shape(P, N, g, s);
+            shaped = true;
+        }
+        if (!view_adjoint(P, N, g, s, cams[v], Gv, gp, gsym, gch, basis)) continue;
+        touched = true;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) sp[c] += gp[c];
+        float gs[7];
+        shape_adjoint(s, gsym, gs);
+#pragma unroll
+        for (int k = 0; k < 7; ++k) sgs[k] += gs[k];
+        sop += Gv[5];
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) {
+            sch[ch] += gch[ch];
+#pragma unroll
+            for (int k = 0; k < 15; ++k) srest[15 * ch + k] = __builtin_fmaf(basis[k], gch[ch], srest[15 * ch + k]);
+        }
+    }
+    if (!touched) {
+        for (int ch = 0; ch < (int)kParams; ++ch) grad[(size_t)ch * N + g] = 0.0f;
+        return;
+    }
+#pragma unroll
+    for (int k = 0; k < 7; ++k) grad[(size_t)(CH_SCALE + k) * N + g] = sgs[k];
+    const float op = opacity_of(P, N, g);
+    grad[(size_t)CH_OPACITY * N + g] = sop * op * (1.0f - op);
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+        grad[(size_t)(CH_DC + ch) * N + g] = SH_C0 * sch[ch];
+#pragma unroll
+        for (int k = 0; k < 15; ++k) grad[(size_t)(CH_REST + 15 * ch + k) * N + g] = srest[15 * ch + k];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) grad[(size_t)(CH_POS + c) * N + g] = sp[c];
 }
 
 }  // namespace
@@ -497,6 +670,52 @@ int launch_backward(Context* c, const float* params, uint32_t count, const CamK&
     GSR_HIP(c, hipGetLastError());
     mark_end(c, false);
     if (grad2d_out) GSR_HIP(c, hipMemcpyAsync(grad2d_out, g2, N * 48, hipMemcpyDeviceToDevice, c->stream));
+    return GSR_OK;
+}
+
+int launch_backward_batch(Context* c, const float* params, uint32_t count, uint32_t views, uint32_t tile,
+                          uint32_t total_tiles, const CamK* cams, const uint4* info, const uint64_t* pix,
+                          int sorted_index, const float* dLdC, float* grad, float* grad2d_out) {
+    const size_t NV = (size_t)count * views;
+    float* g2 = (float*)buffer(c, "b_grad2d", NV * 48);
+    uint32_t* order = (uint32_t*)buffer(c, "b_tile_order", (size_t)total_tiles * 4);
+    if (!g2 || !order) return GSR_ERR_ALLOC;
+    int rc = launch_fill_u32(c, (uint32_t*)g2, 0u, NV * 12);
+    if (rc) return rc;
+    const uint32_t* vals = (const uint32_t*)buffer(c, sorted_index ? "b_vals1" : "b_vals0", 0);
+    const uint2* ranges = (const uint2*)buffer(c, "b_ranges", 0);
+    const SplatRec* rec = (const SplatRec*)buffer(c, "b_splat_rec", 0);
+    const float* tfinal = (const float*)buffer(c, "b_tfinal", 0);
+    const uint32_t* last = (const uint32_t*)buffer(c, "b_last", 0);
+    const FrameCounters* counters = (const FrameCounters*)buffer(c, "b_counters", 0);
+    mark(c, "backward raster");
+    if ((rc = launch_tile_order(c, ranges, total_tiles, order))) return rc;
+    const uint32_t blocks = ceil_div(total_tiles * (tile * tile / 64), 4u);
+    switch (tile) {
+        case 8:
+            hipLaunchKernelGGL(k_raster_backward_batch<8>, dim3(blocks), dim3(256), 0, c->stream, info, pix, views,
+                               total_tiles, (uint32_t)NV, &counters->n, ranges, vals, rec, tfinal, last, dLdC, g2,
+                               order);
+            break;
+        case 16:
+            hipLaunchKernelGGL(k_raster_backward_batch<16>, dim3(blocks), dim3(256), 0, c->stream, info, pix, views,
+                               total_tiles, (uint32_t)NV, &counters->n, ranges, vals, rec, tfinal, last, dLdC, g2,
+                               order);
+            break;
+        case 32:
+            hipLaunchKernelGGL(k_raster_backward_batch<32>, dim3(blocks), dim3(256), 0, c->stream, info, pix, views,
+                               total_tiles, (uint32_t)NV, &counters->n, ranges, vals, rec, tfinal, last, dLdC, g2,
+                               order);
+            break;
+        default:
+            return fail(c, GSR_ERR_STATE, "bad saved tile size %u", tile);
+    }
+    GSR_HIP(c, hipGetLastError());
+    mark(c, "backward preprocess");
+    hipLaunchKernelGGL(k_preprocess_backward_batch, dim3(ceil_div(count, 256)), dim3(256), 0, c->stream, params, count,
+                       cams, views, rec, (const float4*)g2, (float4*)grad2d_out, grad);
+    GSR_HIP(c, hipGetLastError());
+    mark_end(c, false);
     return GSR_OK;
 }
 

@@ -15,6 +15,7 @@
 //     sequence can be captured once as a hipGraph and replayed for new cameras
 //     of the same shape.
 #include <algorithm>
+#include <cstddef>
 #include <cstring>
 
 #include "gsr_internal.h"
@@ -24,6 +25,7 @@ namespace gsr {
 struct BatchPlan {
     uint32_t views = 0, tile = 16, total_tiles = 0, key_bytes = 8, end_bit = 32;
     uint32_t raster_waves = 16, sort_blocks = 2;
+    bool save = false;             // keep T_final / last contributor for gsr_backward_batch
     uint64_t capacity = 0, floats = 0;
     std::vector<uint4> info;       // {W, H, tiles_x, tile_base}
     std::vector<uint64_t> pix;     // float offset of each view's image
@@ -35,6 +37,7 @@ static int plan_batch(Context* c, uint32_t count, const gsr_camera* cams, uint32
     if ((uint64_t)count * views >= 0xFFFFFFFFull)
         return fail(c, GSR_ERR_ARGUMENT, "%u splats x %u views exceed 2^32 entries", count, views);
     p->views = views;
+    p->save = opts && opts->save_state;
     p->tile = (opts && opts->tile_size) ? opts->tile_size : 16;
     if (p->tile != 8 && p->tile != 16 && p->tile != 32)
         return fail(c, GSR_ERR_ARGUMENT, "tile size %u not in {8, 16, 32}", p->tile);
@@ -129,6 +132,13 @@ static int enqueue_batch(Context* c, const float* params, uint32_t count, const 
     if (!tables || !tiles || !offsets || !meta || !rec || !k0 || !k1 || !v0 || !v1 || !ranges ||
         !counters)
         return GSR_ERR_ALLOC;
+    float* tfinal = nullptr;
+    uint32_t* last = nullptr;
+    if (p.save) {
+        tfinal = (float*)buffer(c, "b_tfinal", p.floats / 3 * 4);
+        last = (uint32_t*)buffer(c, "b_last", p.floats / 3 * 4);
+        if (!tfinal || !last) return GSR_ERR_ALLOC;
+    }
     c->scratch_prefix = "b_";
     struct Restore {
         Context* c;
@@ -165,7 +175,7 @@ static int enqueue_batch(Context* c, const float* params, uint32_t count, const 
     if ((rc = launch_ranges(c, ks, p.key_bytes, &counters->n, 0, p.total_tiles, ranges, false))) return rc;
     mark(c, "raster");
     if ((rc = launch_raster_queue(c, p.tile, info_dev, pix_dev, p.views, p.total_tiles, (uint32_t)NV, &counters->n,
-                                  &counters->raster_ticket, ranges, vs, rec, out, p.raster_waves)))
+                                  &counters->raster_ticket, ranges, vs, rec, out, p.raster_waves, tfinal, last)))
         return rc;
     mark_end(c);
     return GSR_OK;
@@ -174,7 +184,7 @@ static int enqueue_batch(Context* c, const float* params, uint32_t count, const 
 static bool same_shape(const BatchGraph& g, const BatchPlan& p, uint32_t count, const float* params, const float* out) {
     if (!g.exec || g.count != count || g.params != params || g.out != out || g.views != p.views || g.tile != p.tile ||
         g.key_bytes != p.key_bytes || g.capacity != p.capacity || g.raster_waves != p.raster_waves ||
-        g.sort_blocks != p.sort_blocks || g.dims.size() != p.views)
+        g.sort_blocks != p.sort_blocks || g.save != p.save || g.dims.size() != p.views)
         return false;
     for (uint32_t v = 0; v < p.views; ++v)
         if (g.dims[v] != ((uint64_t)p.info[v].x << 32 | p.info[v].y)) return false;
@@ -215,15 +225,32 @@ static int capture_batch(Context* c, const float* params, uint32_t count, const 
     g.capacity = p.capacity;
     g.raster_waves = p.raster_waves;
     g.sort_blocks = p.sort_blocks;
+    g.save = p.save;
     g.dims.resize(p.views);
     for (uint32_t v = 0; v < p.views; ++v) g.dims[v] = (uint64_t)p.info[v].x << 32 | p.info[v].y;
     return GSR_OK;
+}
+
+static void save_batch_state(Context* c, const BatchPlan& p, uint32_t count, const gsr_camera* cams) {
+    BatchState& s = c->bstate;
+    s.valid = true;
+    s.count = count;
+    s.views = p.views;
+    s.tile = p.tile;
+    s.total_tiles = p.total_tiles;
+    s.sorted_index = (int)(ceil_div(p.end_bit, kRadixBits) & 1u);  // launch_sort's result index
+    s.cams.resize(p.views);
+    for (uint32_t v = 0; v < p.views; ++v) {
+        std::memcpy(&s.cams[v], &cams[v], sizeof(CamK));
+        s.cams[v].count = count;
+    }
 }
 
 // Runs one batch frame (graph replay when the shape matches) and checks it;
 // on instance overflow the capacity is raised and the frame re-run once.
 static int run_batch(Context* c, const float* params, uint32_t count, const gsr_camera* cams, uint32_t views,
                      const gsr_options* opts, int use_graph, float* out, uint64_t* out_offsets) {
+    if (out) c->bstate.valid = false;  // the batch buffers are about to be overwritten
     for (int attempt = 0; attempt < 2; ++attempt) {
         BatchPlan p;
         int rc = plan_batch(c, count, cams, views, opts, &p);
@@ -245,7 +272,9 @@ static int run_batch(Context* c, const float* params, uint32_t count, const gsr_
         FrameCounters* counters = (FrameCounters*)buffer(c, "b_counters", sizeof(FrameCounters));
         GSR_HIP(c, hipMemcpyAsync(c->host_counters, counters, sizeof(FrameCounters), hipMemcpyDeviceToHost, c->stream));
         GSR_HIP(c, hipStreamSynchronize(c->stream));
-        finish_frame_timings(c);
+        // with saved state the backward's stages join this frame's rows
+        // (collected by gsr_get_timings)
+        if (!p.save) finish_frame_timings(c);
         if (c->host_counters->error)
             return fail(c, GSR_ERR_KERNEL, "a look-back wait exceeded its bound (code %u)", c->host_counters->error);
         const uint64_t inst = c->host_counters->instances;
@@ -254,6 +283,7 @@ static int run_batch(Context* c, const float* params, uint32_t count, const gsr_
             if (use_graph && !replay) {
                 if ((rc = capture_batch(c, params, count, p, out))) return rc;
             }
+            if (p.save) save_batch_state(c, p, count, cams);
             return GSR_OK;
         }
         // overflow: grow (unless the caller fixed the capacity) and re-run
@@ -298,6 +328,7 @@ int gsr_autotune_batch(gsr_context* ctx, const float* params_dev, uint32_t count
     if (!c) return fail(nullptr, GSR_ERR_ARGUMENT, "null context");
     if (!inout || !out_rgb_dev) return fail(c, GSR_ERR_ARGUMENT, "gsr_autotune_batch: null argument");
     hipSetDevice(c->device);
+    c->bstate.valid = false;
     const uint32_t tiles[3] = {8, 16, 32};
     const uint32_t waves[3] = {8, 16, 32};
     const uint32_t blocks[3] = {1, 2, 4};
@@ -347,6 +378,32 @@ int gsr_autotune_batch(gsr_context* ctx, const float* params_dev, uint32_t count
     *inout = best_opts;
     if (best_ms) *best_ms = best;
     return GSR_OK;
+}
+
+int gsr_backward_batch(gsr_context* ctx, const float* params_dev, uint32_t count, const gsr_camera* cams,
+                       uint32_t views, const float* dL_drgb_dev, float* grad_params_dev, float* grad2d_dev) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    if (!c) return fail(nullptr, GSR_ERR_ARGUMENT, "null context");
+    if (!params_dev || !cams || !dL_drgb_dev || !grad_params_dev)
+        return fail(c, GSR_ERR_ARGUMENT, "gsr_backward_batch: null argument");
+    const BatchState& s = c->bstate;
+    if (!s.valid || s.count != count || s.views != views)
+        return fail(c, GSR_ERR_STATE, "gsr_backward_batch needs the matching gsr_render_batch with save_state = 1 first");
+    for (uint32_t v = 0; v < views; ++v) {
+        CamK k;
+        std::memcpy(&k, &cams[v], sizeof(k));
+        k.count = count;
+        if (std::memcmp(&k, &s.cams[v], offsetof(CamK, pad)) != 0)
+            return fail(c, GSR_ERR_STATE, "view %u: camera differs from the saved batch forward", v);
+    }
+    hipSetDevice(c->device);
+    uint8_t* tables = (uint8_t*)buffer(c, "b_tables", 0);
+    if (!tables) return GSR_ERR_ALLOC;
+    const CamK* cams_dev = (const CamK*)tables;
+    const uint4* info_dev = (const uint4*)(tables + views * sizeof(CamK));
+    const uint64_t* pix_dev = (const uint64_t*)(tables + views * (sizeof(CamK) + sizeof(uint4)));
+    return launch_backward_batch(c, params_dev, count, views, s.tile, s.total_tiles, cams_dev, info_dev, pix_dev,
+                                 s.sorted_index, dL_drgb_dev, grad_params_dev, grad2d_dev);
 }
 
 }  // extern "C"

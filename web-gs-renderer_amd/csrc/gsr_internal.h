@@ -136,9 +136,18 @@ struct BatchGraph {
     hipGraphExec_t exec = nullptr;
     uint32_t count = 0, views = 0, tile = 0, key_bytes = 0, raster_waves = 0, sort_blocks = 0;
     uint64_t capacity = 0;
+    bool save = false;
     const float* params = nullptr;
     const float* out = nullptr;
     std::vector<uint64_t> dims;  // (W << 32) | H per view
+};
+
+// What a batch frame with save_state left for gsr_backward_batch.
+struct BatchState {
+    bool valid = false;
+    uint32_t count = 0, views = 0, tile = 16, total_tiles = 0;
+    int sorted_index = 0;          // which of the b_vals buffers holds the sorted entries
+    std::vector<CamK> cams;        // the frame's cameras (device copy at the head of b_tables)
 };
 
 struct Context {
@@ -170,6 +179,7 @@ struct Context {
     size_t staging_bytes = 0;
     uint64_t batch_capacity_hint = 0, batch_last_instances = 0;
     BatchGraph graph;
+    BatchState bstate;
     // host pinned staging for small readbacks
     FrameCounters* host_counters = nullptr;
 };
@@ -242,10 +252,18 @@ struct FillRegion {
 };
 int launch_zero_regions(Context* c, const FillRegion* regions, int count);
 // Batch raster: views[v] = {W, H, tiles_x, tile_base}, pix[v] = float offset of view v's image.
+// tfinal / last (both or neither): per-pixel state, view v's at pix[v] / 3.
 int launch_raster_queue(Context* c, uint32_t tile, const uint4* views, const uint64_t* pix, uint32_t num_views,
                         uint32_t total_tiles, uint32_t splats, const uint32_t* n_dev, uint32_t* ticket,
                         const uint32_t* ranges, const uint32_t* vals, const SplatRec* rec, float* out,
-                        uint32_t waves_per_cu);
+                        uint32_t waves_per_cu, float* tfinal = nullptr, uint32_t* last = nullptr);
+// View batches after a batch forward with saved state: entry v * count + g
+// of the batch buffers ("b_" prefix); dLdC in the batch output layout;
+// gradients summed over views into grad; grad2d_out (nullable) gets the 2-D
+// gradient of every entry.
+int launch_backward_batch(Context* c, const float* params, uint32_t count, uint32_t views, uint32_t tile,
+                          uint32_t total_tiles, const CamK* cams, const uint4* info, const uint64_t* pix,
+                          int sorted_index, const float* dLdC, float* grad, float* grad2d_out);
 
 __host__ __device__ inline uint32_t ceil_div(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 

@@ -240,14 +240,16 @@ __global__ __launch_bounds__(256) void k_raster(uint32_t W, uint32_t H, uint32_t
 // every (view, tile), tile-major; a workgroup of 4 waves takes 4 consecutive
 // items per ticket (one 16-pixel tile, four 8-pixel tiles or a quarter of a
 // 32-pixel tile), one per wave, until all are done.  views[v] = {W, H,
-// tiles_x, tile_base} and pix[v] the float offset of view v's image.
-template <int TS>
+// tiles_x, tile_base} and pix[v] the float offset of view v's image.  SAVE:
+// T_final and the last contributor too, view v's at pixel offset pix[v] / 3.
+template <int TS, bool SAVE>
 __global__ __launch_bounds__(256) void k_raster_queue(const uint4* __restrict__ views,
                                                       const uint64_t* __restrict__ pix, uint32_t num_views,
                                                       uint32_t total_tiles, uint32_t splats, const uint32_t* n_dev,
                                                       uint32_t* ticket, const uint2* __restrict__ ranges,
                                                       const uint32_t* __restrict__ vals,
-                                                      const SplatRec* __restrict__ rec, float* __restrict__ out) {
+                                                      const SplatRec* __restrict__ rec, float* __restrict__ out,
+                                                      float* __restrict__ tfinal, uint32_t* __restrict__ last) {
     constexpr uint32_t SIDE = TS / 8, BLOCKS = SIDE * SIDE;
     __shared__ uint32_t s_ticket[2];
     const uint32_t n_inst = *n_dev;
@@ -273,8 +275,10 @@ __global__ __launch_bounds__(256) void k_raster_queue(const uint4* __restrict__ 
         const uint32_t local = t - vi.w;
         const uint32_t tx = local % vi.z, ty = local / vi.z;
         const uint2 r = ranges[t];
-        raster_block<false>(vi.x, vi.y, tx * TS + (sub % SIDE) * 8, ty * TS + (sub / SIDE) * 8, r.x,
-                            min(r.y, n_inst), splats, vals, rec, out + pix[lo], nullptr, nullptr);
+        const uint64_t po = pix[lo];
+        raster_block<SAVE>(vi.x, vi.y, tx * TS + (sub % SIDE) * 8, ty * TS + (sub / SIDE) * 8, r.x,
+                           min(r.y, n_inst), splats, vals, rec, out + po, SAVE ? tfinal + po / 3 : nullptr,
+                           SAVE ? last + po / 3 : nullptr);
     }
 }
 
@@ -329,27 +333,42 @@ int launch_ranges(Context* c, const void* keys, uint32_t key_bytes, const uint32
     return GSR_OK;
 }
 
+template <int TS>
+static void launch_queue_tile(Context* c, uint32_t groups, const uint4* views, const uint64_t* pix,
+                              uint32_t num_views, uint32_t total_tiles, uint32_t splats, const uint32_t* n_dev,
+                              uint32_t* ticket, const uint2* rg, const uint32_t* vals, const SplatRec* rec, float* out,
+                              float* tfinal, uint32_t* last) {
+    if (tfinal && last)
+        hipLaunchKernelGGL((k_raster_queue<TS, true>), dim3(groups), dim3(256), 0, c->stream, views, pix, num_views,
+                           total_tiles, splats, n_dev, ticket, rg, vals, rec, out, tfinal, last);
+    else
+        hipLaunchKernelGGL((k_raster_queue<TS, false>), dim3(groups), dim3(256), 0, c->stream, views, pix, num_views,
+                           total_tiles, splats, n_dev, ticket, rg, vals, rec, out, nullptr, nullptr);
+}
+
 int launch_raster_queue(Context* c, uint32_t tile, const uint4* views, const uint64_t* pix, uint32_t num_views,
                         uint32_t total_tiles, uint32_t splats, const uint32_t* n_dev, uint32_t* ticket,
                         const uint32_t* ranges, const uint32_t* vals, const SplatRec* rec, float* out,
-                        uint32_t waves_per_cu) {
+                        uint32_t waves_per_cu, float* tfinal, uint32_t* last) {
     if (total_tiles == 0) return GSR_OK;
+    if ((tfinal != nullptr) != (last != nullptr))
+        return fail(c, GSR_ERR_ARGUMENT, "batch raster: T_final and last contributor go together");
     const uint32_t tickets = ceil_div(total_tiles * (tile * tile / 64), 4u);
     const uint32_t groups =
         std::max<uint32_t>(1, std::min<uint32_t>(tickets, (uint32_t)c->cu_count * (waves_per_cu ? waves_per_cu : 16) / 4));
     const uint2* rg = (const uint2*)ranges;
     switch (tile) {
         case 8:
-            hipLaunchKernelGGL(k_raster_queue<8>, dim3(groups), dim3(256), 0, c->stream, views, pix, num_views,
-                               total_tiles, splats, n_dev, ticket, rg, vals, rec, out);
+            launch_queue_tile<8>(c, groups, views, pix, num_views, total_tiles, splats, n_dev, ticket, rg, vals, rec,
+                                 out, tfinal, last);
             break;
         case 16:
-            hipLaunchKernelGGL(k_raster_queue<16>, dim3(groups), dim3(256), 0, c->stream, views, pix, num_views,
-                               total_tiles, splats, n_dev, ticket, rg, vals, rec, out);
+            launch_queue_tile<16>(c, groups, views, pix, num_views, total_tiles, splats, n_dev, ticket, rg, vals, rec,
+                                  out, tfinal, last);
             break;
         case 32:
-            hipLaunchKernelGGL(k_raster_queue<32>, dim3(groups), dim3(256), 0, c->stream, views, pix, num_views,
-                               total_tiles, splats, n_dev, ticket, rg, vals, rec, out);
+            launch_queue_tile<32>(c, groups, views, pix, num_views, total_tiles, splats, n_dev, ticket, rg, vals, rec,
+                                  out, tfinal, last);
             break;
         default:
             return fail(c, GSR_ERR_ARGUMENT, "tile size %u not in {8, 16, 32}", tile);

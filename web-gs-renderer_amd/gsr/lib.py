@@ -106,6 +106,7 @@ def load():
     L.gsr_render_batch.argtypes = [_VP, _VP, _U32, ctypes.POINTER(Camera), _U32, ctypes.POINTER(Options), _INT,
                                    _VP, ctypes.POINTER(_U64)]
     L.gsr_batch_instance_count.argtypes = [_VP, ctypes.POINTER(_U64)]
+    L.gsr_backward_batch.argtypes = [_VP, _VP, _U32, ctypes.POINTER(Camera), _U32, _VP, _VP, _VP]
     L.gsr_autotune_batch.argtypes = [_VP, _VP, _U32, ctypes.POINTER(Camera), _U32, ctypes.POINTER(Options), _VP,
                                      ctypes.POINTER(ctypes.c_double)]
     special = {"gsr_destroy": None, "gsr_last_error": ctypes.c_char_p, "gsr_stream": _VP}
@@ -218,6 +219,12 @@ class Context:
                                             ctypes.byref(opts) if opts else None, 1 if use_graph else 0,
                                             _ptr(out_dev), offs))
         return list(offs)
+
+    def backward_batch(self, params_dev, count: int, uniforms: list, dl_drgb_dev, grad_dev, grad2d_dev=None):
+        """Gradients of the last render_batch (save_state) summed over its views."""
+        cams = (Camera * len(uniforms))(*[Camera.from_uniform(u) for u in uniforms])
+        self._check(self.L.gsr_backward_batch(self.h, _ptr(params_dev), count, cams, len(uniforms),
+                                              _ptr(dl_drgb_dev), _ptr(grad_dev), _ptr(grad2d_dev)))
 
     def batch_layout(self, count: int, uniforms: list, opts: Options | None = None):
         """Float offsets of each view's image in a batch output (views + 1 entries)."""
