@@ -334,7 +334,8 @@ __device__ __forceinline__ void splat_shape(const float* __restrict__ P, size_t 
 //   gch   dL/d(SH colour) per channel (zero where the forward clamped it);
 //   basis the 15 higher SH basis values of the view direction;
 // and the view direction's part of dL/dposition is added to gp.
-__device__ __forceinline__ bool view_adjoint(const float* __restrict__ P, size_t N, uint32_t g, const SplatShape& s,
+template <typename Shape>
+__device__ __forceinline__ bool view_adjoint(const float* __restrict__ P, size_t N, uint32_t g, const Shape& s,
                                              const CamK& cam, const float (&Gv)[9], float (&gp)[3],
                                              float (&gsym)[3][3], float (&gch)[3], float (&basis)[15]) {
     const float px = s.px, py = s.py, pz = s.pz;
@@ -553,13 +554,30 @@ __global__ __launch_bounds__(256) void k_preprocess_backward(const float* __rest
     for (int c = 0; c < 3; ++c) grad[(size_t)(CH_POS + c) * N + g] = gp[c];
 }
 
+// The part of SplatShape the per-view adjoint reads.
+struct SplatCov {
+    float px, py, pz;
+    float S[3][3];
+};
+
+// An opaque copy of a pointer: loads through it are not hoisted out of the
+// loop it is made in or merged with loads through the original, so values
+// the compiler would otherwise keep in registers across a long loop are
+// re-read (from cache) where they are used.
+template <typename T>
+__device__ __forceinline__ const T* opaque(const T* p) {
+    asm volatile("" : "+s"(p));
+    return p;
+}
+
 // View batches: one thread per splat walks the views, converting each
 // entry's sums (entry v * n + g of grad2d, conic from rec at the same index)
-// and summing the parameter gradient over views: each view's gradient is
-// formed as in k_preprocess_backward and added; the opacity and colour ones
-// are linear in the view's 2-D gradient and are summed before the last
-// (view-independent) factor.  grad2d_out (nullable) receives every entry in
-// the 2-D gradient layout.
+// and summing the parameter gradient over views.  Every gradient is linear
+// in the view's 2-D gradient given the view's forward decisions, so the sums
+// are taken before the view-independent factors: dL/dS + (dL/dS)^T (6
+// values) before the scale / rotation chain, dL/d opacity before the sigmoid
+// slope, dL/d colour before the SH-DC constant.  grad2d_out (nullable)
+// receives every entry in the 2-D gradient layout.
 __global__ __launch_bounds__(256) void k_preprocess_backward_batch(const float* __restrict__ P, uint32_t n,
                                                                    const CamK* __restrict__ cams, uint32_t views,
                                                                    const SplatRec* __restrict__ rec,
@@ -569,9 +587,9 @@ __global__ __launch_bounds__(256) void k_preprocess_backward_batch(const float* 
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
     const size_t N = n;
-    SplatShape s;
+    SplatCov cv;
     bool shaped = false, touched = false;
-    float sp[3] = {0.f, 0.f, 0.f}, sgs[7] = {}, sop = 0.0f, sch[3] = {0.f, 0.f, 0.f}, srest[45] = {};
+    float sp[3] = {0.f, 0.f, 0.f}, ss[6] = {}, sop = 0.0f, sch[3] = {0.f, 0.f, 0.f}, srest[45] = {};
     for (uint32_t v = 0; v < views; ++v) {
         const size_t e = (size_t)v * N + g;
         const float4 G0 = grad2d[e * 3], G1 = grad2d[e * 3 + 1], G2 = grad2d[e * 3 + 2];
@@ -584,20 +602,31 @@ __global__ __launch_bounds__(256) void k_preprocess_backward_batch(const float* 
             grad2d_out[e * 3 + 1] = make_float4(Gv[4], Gv[5], Gv[6], Gv[7]);
             grad2d_out[e * 3 + 2] = make_float4(Gv[8], 0.0f, 0.0f, 0.0f);
         }
-        float gp[3], gsym[3][3], gch[3], basis[15];
         if (!any) continue;
+        const float* Pv = opaque(P);  // SH coefficients re-read per view
         if (!shaped) {
-            splat_shape(P, N, g, s);
+            SplatShape full;
+            splat_shape(Pv, N, g, full);
+            cv.px = full.px;
+            cv.py = full.py;
+            cv.pz = full.pz;
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) cv.S[i][j] = full.S[i][j];
             shaped = true;
         }
-        if (!view_adjoint(P, N, g, s, cams[v], Gv, gp, gsym, gch, basis)) continue;
+        float gp[3], gsym[3][3], gch[3], basis[15];
+        if (!view_adjoint(Pv, N, g, cv, cams[v], Gv, gp, gsym, gch, basis)) continue;
         touched = true;
 #pragma unroll
         for (int c = 0; c < 3; ++c) sp[c] += gp[c];
-        float gs[7];
-        shape_adjoint(s, gsym, gs);
-#pragma unroll
-        for (int k = 0; k < 7; ++k) sgs[k] += gs[k];
+        ss[0] += gsym[0][0];
+        ss[1] += gsym[0][1];
+        ss[2] += gsym[0][2];
+        ss[3] += gsym[1][1];
+        ss[4] += gsym[1][2];
+        ss[5] += gsym[2][2];
         sop += Gv[5];
 #pragma unroll
         for (int ch = 0; ch < 3; ++ch) {
@@ -610,9 +639,15 @@ __global__ __launch_bounds__(256) void k_preprocess_backward_batch(const float* 
         for (int ch = 0; ch < (int)kParams; ++ch) grad[(size_t)ch * N + g] = 0.0f;
         return;
     }
+    const float* Pe = opaque(P);  // the shape again, not kept live through the loop
+    SplatShape s;
+    splat_shape(Pe, N, g, s);
+    const float sym[3][3] = {{ss[0], ss[1], ss[2]}, {ss[1], ss[3], ss[4]}, {ss[2], ss[4], ss[5]}};
+    float gs[7];
+    shape_adjoint(s, sym, gs);
 #pragma unroll
-    for (int k = 0; k < 7; ++k) grad[(size_t)(CH_SCALE + k) * N + g] = sgs[k];
-    const float op = opacity_of(P, N, g);
+    for (int k = 0; k < 7; ++k) grad[(size_t)(CH_SCALE + k) * N + g] = gs[k];  // scale, then rotation
+    const float op = opacity_of(Pe, N, g);
     grad[(size_t)CH_OPACITY * N + g] = sop * op * (1.0f - op);
 #pragma unroll
     for (int ch = 0; ch < 3; ++ch) {
