@@ -71,9 +71,17 @@ def cpu_baseline(sc, u, budget_s: float = 15.0):
         if time.perf_counter() - t0 > budget_s or frames >= 8:
             break
     dt = time.perf_counter() - t0
+    # one more frame on a single thread
+    oracle.lib().orc_set_threads(1)
+    t1 = time.perf_counter()
+    oracle.render(packed, sc.count, u, use_radix=True)
+    single = 1.0 / (time.perf_counter() - t1)
+    oracle.lib().orc_set_threads(threads)
     return {"value": frames / dt, "unit": "frames/s", "cores": threads, "kind": "port",
+            "single_thread_value": round(single, 4),
             "sample": f"{frames} full frames of the same 1M-splat 1080p workload through the oracle's FP32 "
-                      f"C++ pass graph (OpenMP, {threads} threads, stable LSD radix sort)"}, evals[-1]
+                      f"C++ pass graph (OpenMP, {threads} threads, stable LSD radix sort), plus one frame on "
+                      f"1 thread (single_thread_value)"}, evals[-1]
 
 
 def batch_leg(ctx, torch, lib, scenes, reps=10):

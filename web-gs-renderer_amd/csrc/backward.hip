@@ -187,23 +187,25 @@ __device__ __forceinline__ void backward_block(uint32_t W, uint32_t H, uint32_t 
             // Every lane evaluates the terms; lanes that did not blend this
             // splat (or sit on the alpha clamp) select zero multipliers.
             // alpha <= 0.99 keeps the recovered T finite in every lane.
-            if (ballot(hit) == 0) continue;
-            const bool grad_ok = hit & (oe <= kAlphaClamp);
-            // undo the blend (v_rcp_f32: 1 ulp, no IEEE division sequence)
+            // Undo the blend (v_rcp_f32: 1 ulp, no IEEE division sequence);
+            // formed before the skip test so that the colour is read from
+            // LDS together with the rest of the record.
             const float Tb = T * __builtin_amdgcn_rcpf(1.0f - alpha);
             const float gal = __builtin_fmaf(cr - Br, gr, __builtin_fmaf(cg - Bg, gg, (cb - Bb) * gb)) * Tb;
+            asm volatile("" ::"v"(gal));  // keeps gal (and the colour read) ahead of the branch
+            if (ballot(hit) == 0) continue;
+            const bool grad_ok = hit & (oe <= kAlphaClamp);
             const float w = hit ? alpha * Tb : 0.0f;
             const float gpow = grad_ok ? alpha * gal : 0.0f;
             const float gdx = gpow * dx, gdy = gpow * dy;
             const float a[8] = {gdx, gdy, gdx * dx, gdx * dy, gdy * dy, grad_ok ? e * gal : 0.0f, w * gr, w * gg};
             const float a_b = w * gb;
-            if (hit) {
-                T = Tb;
-                const float om = 1.0f - alpha;
-                Br = __builtin_fmaf(alpha, cr, om * Br);
-                Bg = __builtin_fmaf(alpha, cg, om * Bg);
-                Bb = __builtin_fmaf(alpha, cb, om * Bb);
-            }
+            // state before this splat where it was blended (selects, no branch)
+            const float om = 1.0f - alpha;
+            T = hit ? Tb : T;
+            Br = hit ? __builtin_fmaf(alpha, cr, om * Br) : Br;
+            Bg = hit ? __builtin_fmaf(alpha, cg, om * Bg) : Bg;
+            Bb = hit ? __builtin_fmaf(alpha, cb, om * Bb) : Bb;
             float s8, sb;
             fold9(a, a_b, s8, sb);
             const uint32_t g = __float_as_uint(s.cl.w);

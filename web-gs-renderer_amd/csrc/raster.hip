@@ -141,8 +141,11 @@ __device__ __forceinline__ void raster_block(uint32_t W, uint32_t H, uint32_t x0
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         // Only splats that can reach the block, in range order.  The pixel
-        // tests are evaluated as predicates (no per-test branches); they are
-        // the skip/stop conditions above, in the same order of effect.
+        // tests are evaluated as predicates and the blend update as selects,
+        // so the loop body has no branch and reads the whole LDS record with
+        // one wait; the tests are the skip/stop conditions above, in the same
+        // order of effect.  colour += c * 0 where nothing is blended leaves
+        // it unchanged bit for bit (colours are finite and >= 0).
         for (uint64_t lm = ballot(live); lm != 0; lm &= lm - 1) {
             const int j = (int)__builtin_ctzll(lm);
             const SplatL s = sw[j];
@@ -154,14 +157,13 @@ __device__ __forceinline__ void raster_block(uint32_t W, uint32_t H, uint32_t x0
             const bool ok = !done & !(power > 0.0f) & !(power < -5.55f) & !(alpha < kAlphaMin);
             const bool low = tn < kTransmitMin;
             done |= ok & low;
-            if (ok & !low) {
-                const float w = alpha * T;
-                c0 = __builtin_fmaf(s.cl.x, w, c0);
-                c1 = __builtin_fmaf(s.cl.y, w, c1);
-                c2 = __builtin_fmaf(s.cl.z, w, c2);
-                T = tn;
-                if (SAVE) lst = b - start + (uint32_t)j + 1;
-            }
+            const bool use = ok & !low;
+            const float w = use ? alpha * T : 0.0f;
+            c0 = __builtin_fmaf(s.cl.x, w, c0);
+            c1 = __builtin_fmaf(s.cl.y, w, c1);
+            c2 = __builtin_fmaf(s.cl.z, w, c2);
+            T = use ? tn : T;
+            if (SAVE) lst = use ? b - start + (uint32_t)j + 1 : lst;
         }
     }
     if (inside) {
