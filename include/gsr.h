@@ -67,6 +67,9 @@ typedef struct gsr_options {
     uint32_t save_state;       /* 1 = keep per-pixel T_final / last contributor for backward     */
     uint32_t raster_waves_per_cu; /* batches: persistent raster waves per CU; 0 = 16              */
     uint32_t sort_blocks_per_cu;  /* onesweep blocks per CU: 1, 2 or 4 (1024-, 512-, 256-thread); 0 = 1 */
+    uint32_t sort_digit_bits;     /* radix digit width: 8 to 11 bits (256 to 2048 buckets); 0 = 8 for
+                                     32-bit keys, for 64-bit keys the narrowest of 8..10 with the
+                                     fewest passes                                                  */
 } gsr_options;
 
 /* ---- timings --------------------------------------------------------- */
@@ -163,6 +166,15 @@ int gsr_sort_pairs(gsr_context* ctx, uint32_t key_bytes, void* keys0_dev, uint32
                    void* keys1_dev, uint32_t* values1_dev, uint64_t n,
                    uint32_t begin_bit, uint32_t end_bit, int* result_index);
 
+/* gsr_sort_pairs with an explicit onesweep geometry: blocks_per_cu 1, 2 or 4
+ * (1024-, 512- or 256-thread blocks) and digit_bits 8 to 11 (the sort
+ * dimensions of gsr_options / gsr_autotune_batch).  Same result for every
+ * geometry.  No reference counterpart beyond gsr_sort_pairs'. */
+int gsr_sort_pairs_geometry(gsr_context* ctx, uint32_t key_bytes, void* keys0_dev, uint32_t* values0_dev,
+                            void* keys1_dev, uint32_t* values1_dev, uint64_t n,
+                            uint32_t begin_bit, uint32_t end_bit, uint32_t blocks_per_cu,
+                            uint32_t digit_bits, int* result_index);
+
 /* tile_ranges.wgsl:7-35: ranges[2t] = first, ranges[2t+1] = last+1 of tile t
  * (= key >> 16) over sorted keys[0..n); untouched tiles are (0,0). */
 int gsr_tile_ranges(gsr_context* ctx, const void* keys_dev, uint32_t key_bytes, uint64_t n,
@@ -218,7 +230,8 @@ int gsr_batch_instance_count(gsr_context* ctx, uint64_t* instances);
 
 /*
  * Autotuner for a batch shape: times tile size {8, 16, 32} x persistent raster
- * waves per CU {8, 16, 32} x onesweep blocks per CU {1, 2, 4} (three timed
+ * waves per CU {8, 16, 32} x onesweep blocks per CU {1, 2, 4} x radix digit
+ * width {8, 9, 10, 11} bits (three timed
  * frames each, HIP events) and writes the fastest into *inout (other fields
  * kept).  out_rgb_dev is used as the scratch output.  The tile size changes
  * which splats reach a pixel's tile at the fringe of their 3-sigma square, so

@@ -20,7 +20,7 @@ def test_library_exports_header_symbols():
 
 def test_header_camera_block_is_176_bytes():
     assert ctypes.sizeof(lib.Camera) == 176
-    assert ctypes.sizeof(lib.Options) == 32      # u32 u32 u64 u32 u32 u32 u32
+    assert ctypes.sizeof(lib.Options) == 40      # u32 u32 u64 u32 u32 u32 u32 u32 (+4 padding)
 
 
 def _has_gpu():

@@ -68,15 +68,18 @@ def test_batch_views_equal_plain_renders(torch_cuda, ctx, tile):
     assert ctx.batch_instance_count() == total
 
 
-@pytest.mark.parametrize("blocks", [2, 4])
-def test_batch_sort_geometries_agree(torch_cuda, ctx, blocks):
-    """64-bit view|tile|depth keys through onesweep at 2 and 4 blocks per CU
-    give the batch of the default 1 block per CU."""
+@pytest.mark.parametrize("blocks,digits", [(2, 8), (4, 8), (1, 9), (2, 10), (1, 11), (4, 11)])
+def test_batch_sort_geometries_agree(torch_cuda, ctx, blocks, digits):
+    """64-bit view|tile|depth keys through onesweep at 1, 2 and 4 blocks per CU
+    and 8- to 11-bit digits give the batch of the default (1 block per CU,
+    8-bit digits); the saved state for the backward follows the sort's
+    result buffer."""
     sc, us = _scene_and_cams(n=60_000, views=12, seed=13)
     params = to_dev(torch_cuda, sc.planar())
     want, _ = _batch(torch_cuda, ctx, params, sc.count, us, lib.options(tile_size=16, key_bytes=lib.KEY_64))
     got, _ = _batch(torch_cuda, ctx, params, sc.count, us,
-                    lib.options(tile_size=16, key_bytes=lib.KEY_64, sort_blocks_per_cu=blocks))
+                    lib.options(tile_size=16, key_bytes=lib.KEY_64, sort_blocks_per_cu=blocks,
+                                sort_digit_bits=digits))
     for v in range(len(us)):
         assert np.array_equal(got[v], want[v]), v
 
@@ -126,7 +129,7 @@ def test_autotuner_picks_a_valid_configuration(torch_cuda, ctx):
     _say(f"autotuned: tile {opts.tile_size} waves {opts.raster_waves_per_cu} blocks {opts.sort_blocks_per_cu} "
          f"{best:.3f} ms")
     assert opts.tile_size in (8, 16, 32) and opts.raster_waves_per_cu in (8, 16, 32)
-    assert opts.sort_blocks_per_cu in (1, 2, 4) and best > 0
+    assert opts.sort_blocks_per_cu in (1, 2, 4) and opts.sort_digit_bits in (8, 9, 10, 11) and best > 0
     imgs, _ = _batch(torch_cuda, ctx, params, sc.count, us, opts)
     ref = _plain(torch_cuda, ctx, params, sc.count, us[1], opts.tile_size)
     assert np.array_equal(imgs[1], ref)

@@ -90,23 +90,28 @@ def test_render_1m_1080p_matches_oracle(torch_cuda, ctx, orc):
 @pytest.mark.parametrize("key_bytes", [lib.KEY_32, lib.KEY_64])
 def test_sort_geometries_give_the_same_frame(torch_cuda, ctx, orc, key_bytes):
     """Onesweep with 1, 2 and 4 blocks per CU (1024-, 512-, 256-thread blocks,
-    different tile sizes and look-back chains) must give the same frame."""
+    different tile sizes and look-back chains) and 8- to 11-bit digits (4 or
+    3 passes) must give the same frame."""
     sc = scenes.clustered_scene(300_000, seed=12)
     cam = scenes.Camera((0.5, -0.3, -7.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 55.0, 1280, 720)
     u = cam.uniform(sc.count)
     ctx.scene_upload("geo", sc.packed(), sc.count)
     ctx.scene_select("geo")
     imgs = {}
-    for blocks in (1, 2, 4):
-        _say(f"sort geometry: {blocks} blocks/CU, key bytes {key_bytes}")
-        imgs[blocks] = ctx.render(u, lib.options(key_bytes=key_bytes, sort_blocks_per_cu=blocks))
-        assert np.array_equal(imgs[blocks], imgs[1]), blocks
+    for digits in (8, 9, 10, 11):
+        for blocks in (1, 2, 4):
+            _say(f"sort geometry: {blocks} blocks/CU, {digits}-bit digits, key bytes {key_bytes}")
+            imgs[blocks, digits] = ctx.render(u, lib.options(key_bytes=key_bytes, sort_blocks_per_cu=blocks,
+                                                             sort_digit_bits=digits))
+            assert np.array_equal(imgs[blocks, digits], imgs[1, 8]), (blocks, digits)
+    imgs[1] = imgs[1, 8]
     ref, _, _, Iref, _ = orc.render(sc.packed(), sc.count, u, use_radix=True)
     assert ctx.instance_count() == Iref
     assert np.array_equal(imgs[1], ref)
-    with pytest.raises(lib.GsrError) as e:
-        ctx.render(u, lib.options(sort_blocks_per_cu=3))
-    assert e.value.code == lib.GSR_ERR_ARGUMENT
+    for bad in (lib.options(sort_blocks_per_cu=3), lib.options(sort_digit_bits=12)):
+        with pytest.raises(lib.GsrError) as e:
+            ctx.render(u, bad)
+        assert e.value.code == lib.GSR_ERR_ARGUMENT
     ctx.scene_free("geo")
 
 

@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 from conftest import empty_dev, to_dev, to_host
-from gsr import scenes
+from gsr import lib, scenes
 
 pytestmark = pytest.mark.gpu
 
@@ -126,7 +126,7 @@ def test_emit_exact(torch_cuda, ctx, orc, key_bytes):
     assert np.array_equal(got[:cap], keys[:cap]) and not got[cap:].any()
 
 
-def _gpu_sort(torch, ctx, keys, vals, key_bytes, begin, end):
+def _gpu_sort(torch, ctx, keys, vals, key_bytes, begin, end, blocks=None, digits=None):
     n = keys.size
     kdt = np.uint32 if key_bytes == 4 else np.uint64
     k0 = to_dev(torch, keys.astype(kdt) if n else np.zeros(1, kdt))
@@ -134,7 +134,7 @@ def _gpu_sort(torch, ctx, keys, vals, key_bytes, begin, end):
     k1 = empty_dev(torch, n, kdt)
     v1 = empty_dev(torch, n, np.uint32)
     torch.cuda.synchronize()
-    idx = ctx.sort_pairs(key_bytes, k0, v0, k1, v1, n, begin, end)
+    idx = ctx.sort_pairs(key_bytes, k0, v0, k1, v1, n, begin, end, blocks, digits)
     ks = (k1 if idx else k0)
     vs = (v1 if idx else v0)
     return to_host(ks, kdt)[:n].astype(np.uint64), to_host(vs, np.uint32)[:n]
@@ -156,6 +156,36 @@ def test_sort_stable_exact(torch_cuda, ctx, n, key_bytes, bits):
     order = np.argsort(keys, kind="stable")
     assert np.array_equal(k, keys[order])
     assert np.array_equal(v, vals[order])
+
+
+@pytest.mark.parametrize("blocks,digits", [(b, d) for d in (8, 9, 10, 11) for b in (1, 2, 4)])
+@pytest.mark.parametrize("n,key_bytes,bits,begin", [
+    (0, 4, 32, 0), (1, 4, 29, 0), (8193, 8, 33, 0), (300_000, 4, 32, 8),
+    (1_000_003, 4, 29, 0), (2_500_000, 8, 34, 0), (600_000, 8, 64, 0)])
+def test_sort_geometries_stable_exact(torch_cuda, ctx, blocks, digits, n, key_bytes, bits, begin):
+    """Every onesweep geometry (1024/512/256-thread blocks, 8- to 11-bit
+    digits up to 11 bits) gives the stable sort on bits [begin, bits)."""
+    _say(f"sort geometry blocks={blocks} digits={digits} n={n} key_bytes={key_bytes} bits=[{begin},{bits})")
+    rng = np.random.default_rng(n + bits + 7 * digits)
+    hi = (1 << bits) - 1 if bits < 64 else np.iinfo(np.uint64).max
+    keys = rng.integers(0, hi, size=n, dtype=np.uint64, endpoint=True)
+    if n > 10:
+        keys[: n // 2] = keys[rng.integers(0, 8, n // 2)]   # heavy ties
+    vals = rng.permutation(n).astype(np.uint32)
+    k, v = _gpu_sort(torch_cuda, ctx, keys, vals, key_bytes, begin, bits, blocks, digits)
+    width = bits - begin
+    win = (keys >> np.uint64(begin)) & (np.uint64((1 << width) - 1) if width < 64 else np.iinfo(np.uint64).max)
+    order = np.argsort(win, kind="stable")
+    assert np.array_equal(v, vals[order])
+    assert np.array_equal(k, keys[order])
+
+
+def test_sort_geometry_rejects_unknown_widths(torch_cuda, ctx):
+    keys = np.arange(10, dtype=np.uint64)
+    for blocks, digits in ((3, 8), (1, 7), (1, 12)):
+        with pytest.raises(lib.GsrError) as e:
+            _gpu_sort(torch_cuda, ctx, keys, keys.astype(np.uint32), 4, 0, 32, blocks, digits)
+        assert e.value.code == lib.GSR_ERR_ARGUMENT
 
 
 def test_sort_bit_window_and_all_equal(torch_cuda, ctx):

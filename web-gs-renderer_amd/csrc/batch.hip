@@ -24,7 +24,7 @@ namespace gsr {
 
 struct BatchPlan {
     uint32_t views = 0, tile = 16, total_tiles = 0, key_bytes = 8, end_bit = 32;
-    uint32_t raster_waves = 16, sort_blocks = 2;
+    uint32_t raster_waves = 16, sort_blocks = 2, sort_digits = 8;
     bool save = false;             // keep T_final / last contributor for gsr_backward_batch
     uint64_t capacity = 0, floats = 0;
     std::vector<uint4> info;       // {W, H, tiles_x, tile_base}
@@ -72,6 +72,10 @@ static int plan_batch(Context* c, uint32_t count, const gsr_camera* cams, uint32
     if (p->key_bytes == 4 && p->end_bit > 32)
         return fail(c, GSR_ERR_ARGUMENT, "%llu tiles do not fit a 32-bit view|tile|depth key",
                     (unsigned long long)tiles);
+    p->sort_digits = (opts && opts->sort_digit_bits) ? opts->sort_digit_bits
+                                                      : default_digit_bits(p->key_bytes, p->end_bit);
+    if (p->sort_digits < 8 || p->sort_digits > 11)
+        return fail(c, GSR_ERR_ARGUMENT, "sort digit width %u not in 8..11", p->sort_digits);
     uint64_t cap = (opts && opts->instance_capacity) ? opts->instance_capacity : c->batch_capacity_hint;
     if (cap == 0) cap = std::min<uint64_t>(kMaxSortKeys, (uint64_t)16 * count * views);
     if (cap > kMaxSortKeys) {
@@ -167,7 +171,7 @@ static int enqueue_batch(Context* c, const float* params, uint32_t count, const 
     mark(c, "sort");
     int idx = 0;
     if ((rc = launch_sort(c, p.key_bytes, k0, v0, k1, v1, &counters->n, 0, p.capacity, 0, p.end_bit,
-                          p.sort_blocks, counters, &idx)))
+                          p.sort_blocks, counters, &idx, p.sort_digits)))
         return rc;
     mark(c, "tile ranges");
     void* ks = idx ? k1 : k0;
@@ -184,7 +188,8 @@ static int enqueue_batch(Context* c, const float* params, uint32_t count, const 
 static bool same_shape(const BatchGraph& g, const BatchPlan& p, uint32_t count, const float* params, const float* out) {
     if (!g.exec || g.count != count || g.params != params || g.out != out || g.views != p.views || g.tile != p.tile ||
         g.key_bytes != p.key_bytes || g.capacity != p.capacity || g.raster_waves != p.raster_waves ||
-        g.sort_blocks != p.sort_blocks || g.save != p.save || g.dims.size() != p.views)
+        g.sort_blocks != p.sort_blocks || g.sort_digits != p.sort_digits || g.save != p.save ||
+        g.dims.size() != p.views)
         return false;
     for (uint32_t v = 0; v < p.views; ++v)
         if (g.dims[v] != ((uint64_t)p.info[v].x << 32 | p.info[v].y)) return false;
@@ -225,6 +230,7 @@ static int capture_batch(Context* c, const float* params, uint32_t count, const 
     g.capacity = p.capacity;
     g.raster_waves = p.raster_waves;
     g.sort_blocks = p.sort_blocks;
+    g.sort_digits = p.sort_digits;
     g.save = p.save;
     g.dims.resize(p.views);
     for (uint32_t v = 0; v < p.views; ++v) g.dims[v] = (uint64_t)p.info[v].x << 32 | p.info[v].y;
@@ -238,7 +244,7 @@ static void save_batch_state(Context* c, const BatchPlan& p, uint32_t count, con
     s.views = p.views;
     s.tile = p.tile;
     s.total_tiles = p.total_tiles;
-    s.sorted_index = (int)(ceil_div(p.end_bit, kRadixBits) & 1u);  // launch_sort's result index
+    s.sorted_index = (int)sort_result_index(0, p.end_bit, p.sort_digits);
     s.cams.resize(p.views);
     for (uint32_t v = 0; v < p.views; ++v) {
         std::memcpy(&s.cams[v], &cams[v], sizeof(CamK));
@@ -332,6 +338,7 @@ int gsr_autotune_batch(gsr_context* ctx, const float* params_dev, uint32_t count
     const uint32_t tiles[3] = {8, 16, 32};
     const uint32_t waves[3] = {8, 16, 32};
     const uint32_t blocks[3] = {1, 2, 4};
+    const uint32_t digits[4] = {8, 9, 10, 11};
     hipEvent_t e0, e1;
     GSR_HIP(c, hipEventCreate(&e0));
     GSR_HIP(c, hipEventCreate(&e1));
@@ -342,11 +349,12 @@ int gsr_autotune_batch(gsr_context* ctx, const float* params_dev, uint32_t count
     int rc = GSR_OK;
     for (uint32_t ti = 0; ti < 3 && rc == GSR_OK; ++ti)
         for (uint32_t wi = 0; wi < 3 && rc == GSR_OK; ++wi)
-            for (uint32_t bi = 0; bi < 3 && rc == GSR_OK; ++bi) {
+            for (uint32_t bd = 0; bd < 12 && rc == GSR_OK; ++bd) {
                 gsr_options o = *inout;
                 o.tile_size = tiles[ti];
                 o.raster_waves_per_cu = waves[wi];
-                o.sort_blocks_per_cu = blocks[bi];
+                o.sort_blocks_per_cu = blocks[bd % 3];
+                o.sort_digit_bits = digits[bd / 3];
                 o.timing = 0;
                 // a tile size whose view|tile|depth key does not fit the requested key width is skipped
                 BatchPlan p;

@@ -99,9 +99,7 @@ static_assert(sizeof(CamK) == 176, "camera block must be 176 bytes");
 static_assert(sizeof(gsr_camera) == 176, "gsr_camera must be 176 bytes");
 
 // ---- sort tuning ---------------------------------------------------------
-constexpr int kRadixBits = 8;
-constexpr int kRadix = 1 << kRadixBits;
-constexpr int kMaxPasses = 8;
+constexpr int kMaxPasses = 8;                           // 64-bit keys at 8-bit digits
 constexpr uint32_t kStatusValueMask = 0x3FFFFFFFu;    // 30-bit counts in look-back words
 constexpr uint64_t kMaxSortKeys = kStatusValueMask;   // sort capacity limit
 
@@ -134,7 +132,7 @@ struct TimingAcc {
 // A captured batch frame and the shape it was captured for.
 struct BatchGraph {
     hipGraphExec_t exec = nullptr;
-    uint32_t count = 0, views = 0, tile = 0, key_bytes = 0, raster_waves = 0, sort_blocks = 0;
+    uint32_t count = 0, views = 0, tile = 0, key_bytes = 0, raster_waves = 0, sort_blocks = 0, sort_digits = 0;
     uint64_t capacity = 0;
     bool save = false;
     const float* params = nullptr;
@@ -231,10 +229,18 @@ uint32_t* scan_status_buffer(Context* c, uint32_t n);
 int launch_emit(Context* c, const uint32_t* offsets, const uint4* meta, uint32_t n, uint32_t tiles_x,
                 uint64_t capacity, uint32_t key_bytes, void* keys, uint32_t* vals);
 // Sort of n (device counter n_dev if non-null, else n_host) pairs with
-// onesweep blocks_per_cu in {1, 2, 4}; returns the result index.
+// onesweep blocks_per_cu in {1, 2, 4} and digit_bits in 8..11; returns the
+// result index (which ping-pong buffer holds the output).
 int launch_sort(Context* c, uint32_t key_bytes, void* k0, uint32_t* v0, void* k1, uint32_t* v1,
                 const uint32_t* n_dev, uint64_t n_host, uint64_t capacity, uint32_t begin_bit,
-                uint32_t end_bit, uint32_t blocks_per_cu, FrameCounters* counters, int* result_index);
+                uint32_t end_bit, uint32_t blocks_per_cu, FrameCounters* counters, int* result_index,
+                uint32_t digit_bits = 8);
+// The result index launch_sort reports for this bit range and digit width.
+uint32_t sort_result_index(uint32_t begin_bit, uint32_t end_bit, uint32_t digit_bits);
+// Digit width used when the options leave it open: 8 bits for 32-bit keys;
+// for 64-bit keys the narrowest of 8..10 bits with the fewest passes over
+// [0, end_bit) (a 33- or 34-bit view|tile|depth key: 9 bits, 4 passes).
+uint32_t default_digit_bits(uint32_t key_bytes, uint32_t end_bit);
 int launch_ranges(Context* c, const void* keys, uint32_t key_bytes, const uint32_t* n_dev,
                   uint64_t n_host, uint32_t num_tiles, uint32_t* ranges, bool clear);
 // splats bounds the per-splat records, the instance count (n_dev, else n_host) the ranges.

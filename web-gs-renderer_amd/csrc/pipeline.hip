@@ -12,7 +12,7 @@
 namespace gsr {
 
 struct FramePlan {
-    uint32_t W, H, tile, tiles_x, tiles_y, num_tiles, key_bytes, end_bit, sort_blocks;
+    uint32_t W, H, tile, tiles_x, tiles_y, num_tiles, key_bytes, end_bit, sort_blocks, sort_digits;
     uint64_t capacity;
 };
 
@@ -50,6 +50,10 @@ int plan_frame(Context* c, uint32_t count, const gsr_camera* cam, const gsr_opti
     p->sort_blocks = (opts && opts->sort_blocks_per_cu) ? opts->sort_blocks_per_cu : 1;
     if (p->sort_blocks != 1 && p->sort_blocks != 2 && p->sort_blocks != 4)
         return fail(c, GSR_ERR_ARGUMENT, "sort blocks per CU %u not in {1, 2, 4}", p->sort_blocks);
+    p->sort_digits = (opts && opts->sort_digit_bits) ? opts->sort_digit_bits
+                                                      : default_digit_bits(p->key_bytes, p->end_bit);
+    if (p->sort_digits < 8 || p->sort_digits > 11)
+        return fail(c, GSR_ERR_ARGUMENT, "sort digit width %u not in 8..11", p->sort_digits);
     return GSR_OK;
 }
 
@@ -96,7 +100,7 @@ int enqueue_forward(Context* c, const float* params, uint32_t count, const gsr_c
     mark(c, "sort");
     int idx = 0;
     if ((rc = launch_sort(c, p.key_bytes, k0, v0, k1, v1, &counters->n, 0, p.capacity, 0, p.end_bit,
-                          p.sort_blocks, counters, &idx)))
+                          p.sort_blocks, counters, &idx, p.sort_digits)))
         return rc;
     mark(c, "tile ranges");
     void* ks = idx ? k1 : k0;
@@ -255,8 +259,9 @@ int gsr_emit(gsr_context* ctx, const uint32_t* offsets_dev, const uint32_t* meta
     return GSR_OK;
 }
 
-int gsr_sort_pairs(gsr_context* ctx, uint32_t key_bytes, void* keys0_dev, uint32_t* values0_dev, void* keys1_dev,
-                   uint32_t* values1_dev, uint64_t n, uint32_t begin_bit, uint32_t end_bit, int* result_index) {
+int gsr_sort_pairs_geometry(gsr_context* ctx, uint32_t key_bytes, void* keys0_dev, uint32_t* values0_dev,
+                            void* keys1_dev, uint32_t* values1_dev, uint64_t n, uint32_t begin_bit, uint32_t end_bit,
+                            uint32_t blocks_per_cu, uint32_t digit_bits, int* result_index) {
     Context* c = ctx_of(ctx);
     if (!c) return fail(nullptr, GSR_ERR_ARGUMENT, "null context");
     if (!keys0_dev || !values0_dev || !keys1_dev || !values1_dev || !result_index)
@@ -268,12 +273,18 @@ int gsr_sort_pairs(gsr_context* ctx, uint32_t key_bytes, void* keys0_dev, uint32
     if (!counters) return GSR_ERR_ALLOC;
     int rc = launch_fill_u32(c, (uint32_t*)counters, 0u, sizeof(FrameCounters) / 4);
     if (!rc) rc = launch_sort(c, key_bytes, keys0_dev, values0_dev, keys1_dev, values1_dev, nullptr, n,
-                              n ? n : 1, begin_bit, end_bit, 1, counters, result_index);
+                              n ? n : 1, begin_bit, end_bit, blocks_per_cu, counters, result_index, digit_bits);
     if (rc) return rc;
     GSR_HIP(c, hipMemcpyAsync(c->host_counters, counters, sizeof(FrameCounters), hipMemcpyDeviceToHost, c->stream));
     GSR_HIP(c, hipStreamSynchronize(c->stream));
     if (c->host_counters->error) return fail(c, GSR_ERR_KERNEL, "sort look-back wait exceeded its bound");
     return GSR_OK;
+}
+
+int gsr_sort_pairs(gsr_context* ctx, uint32_t key_bytes, void* keys0_dev, uint32_t* values0_dev, void* keys1_dev,
+                   uint32_t* values1_dev, uint64_t n, uint32_t begin_bit, uint32_t end_bit, int* result_index) {
+    return gsr_sort_pairs_geometry(ctx, key_bytes, keys0_dev, values0_dev, keys1_dev, values1_dev, n, begin_bit,
+                                   end_bit, 1, 8, result_index);
 }
 
 int gsr_tile_ranges(gsr_context* ctx, const void* keys_dev, uint32_t key_bytes, uint64_t n, uint32_t num_tiles,

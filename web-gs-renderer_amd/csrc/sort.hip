@@ -6,7 +6,7 @@
 // order of the emitted instances.  Structure per sort:
 //   k_sort_init      zero digit histograms, tile tickets and look-back words
 //                    (only for the tiles the device-side count needs);
-//   k_sort_hist      one read of the keys builds every pass's 256-bin digit
+//   k_sort_hist      one read of the keys builds every pass's digit
 //                    histogram (LDS bins; a run of equal digits in
 //                    consecutive lanes adds once; one global atomic per bin
 //                    and block, one block per CU);
@@ -14,14 +14,19 @@
 //                    wave64 with ballot digit matching (stable), chain
 //                    per-digit tile prefixes with decoupled look-back, stage
 //                    the tile digit-sorted in LDS and write it out in runs.
-// Onesweep geometry: THREADS = 1024 / blocks-per-CU threads (1, 2 or 4
-// blocks per CU) with 16 items per thread for 32-bit keys and 8 for 64-bit
-// keys.  Large tiles keep the look-back chains short: a tile's look-back
-// walks back through the tiles that started with it, 16 per round trip, so
-// the count of tiles in flight sets its latency.
+// Geometry: THREADS = 1024 / blocks-per-CU threads (1, 2 or 4 blocks per
+// CU), digits of 8 to 11 bits (256 to 2048 buckets: a 33- or 34-bit
+// view|tile|depth key takes 4 passes at 9 bits instead of 5 at 8, a 29-bit
+// frame key 3 at 10 or 11 bits), and as many items per thread as the CU's
+// LDS holds (SortGeom).  Large tiles keep the look-back
+// chains short: a tile's look-back walks back through the tiles that
+// started with it, 16 words per round trip, so the count of tiles in flight
+// sets its latency.
 // The key count may live on the device (the frame pipeline), so every grid is
 // sized from the capacity and blocks exit on the device count: no host
 // readback, capturable in a hipGraph.
+#include <type_traits>
+
 #include "gsr_internal.h"
 #include "wave.h"
 
@@ -29,65 +34,112 @@ namespace gsr {
 namespace {
 
 constexpr uint32_t kAgg = 1u << 30, kInc = 2u << 30;
-constexpr int kLookBack = 16;   // predecessor words fetched per look-back step
+constexpr int kLookBack = 16;   // predecessor words fetched per look-back step (all digits of a thread)
 constexpr int kHistItems = 16;  // keys per thread per histogram block tile
+constexpr size_t kLdsBytes = 160 * 1024;
 
-template <typename K, int THREADS>
+// Onesweep geometry for keys K, THREADS threads per block and RB-bit digits.
+// Each thread ranks kItems keys; the tile is THREADS * kItems keys, the
+// largest of 16 / 8 / 4 items whose LDS (keys and values staged, 16-bit
+// per-wave digit counts, three digit tables) fits one CU.  Digits are owned
+// kDPT per thread (thread t: digits t * kDPT ...), so 2048 digits need no
+// more than 512 threads.
+template <typename K, int THREADS, int RB>
 struct SortGeom {
     static constexpr int kThreads = THREADS;
     static constexpr int kWaves = THREADS / 64;
-    static constexpr int kItems = sizeof(K) == 4 ? 16 : 8;
+    static constexpr int kRadixN = 1 << RB;
+    static constexpr int kDPT = kRadixN > THREADS ? kRadixN / THREADS : 1;
+    // per-wave digit counts (at most 64 * items, prefixes at most the tile):
+    // 16 bits when the 2048-bucket tables would not fit LDS otherwise
+    using CntT = typename std::conditional<(RB > 8), uint16_t, uint32_t>::type;
+    static constexpr int kDigitThreads = kRadixN / kDPT;
+    static constexpr size_t lds(int items) {
+        return (size_t)THREADS * items * (sizeof(K) + 4) + (size_t)kWaves * kRadixN * sizeof(CntT) + (size_t)3 * kRadixN * 4 +
+               64 * 4 + 64;
+    }
+    static constexpr int kItems = lds(sizeof(K) == 4 ? 16 : 8) <= kLdsBytes ? (sizeof(K) == 4 ? 16 : 8)
+                                  : lds(sizeof(K) == 4 ? 8 : 4) <= kLdsBytes ? (sizeof(K) == 4 ? 8 : 4)
+                                                                              : (sizeof(K) == 4 ? 4 : 2);
     static constexpr int kTile = THREADS * kItems;
+    static_assert(lds(kItems) <= kLdsBytes, "onesweep geometry does not fit LDS");
+    static constexpr int kLB = kLookBack / kDPT > 4 ? kLookBack / kDPT : 4;  // predecessors per step
+    static constexpr int kDigitWaves = (kDigitThreads + 63) / 64;
 };
+
+// Passes of RB-bit digits over key bits [begin_bit, end_bit).
+__host__ __device__ constexpr uint32_t sort_passes(uint32_t begin_bit, uint32_t end_bit, uint32_t rb) {
+    return end_bit > begin_bit ? (end_bit - begin_bit + rb - 1) / rb : 0;
+}
 
 __device__ __forceinline__ uint32_t read_n(const uint32_t* n_dev, uint64_t n_host) {
     return n_dev ? *n_dev : (uint32_t)n_host;
 }
 
-// Exclusive prefix over threads 0..255 of a block (waves 0-3).  Every thread
-// of the block calls it (threads >= 256 pass 0); s4 holds 4 words.
-__device__ __forceinline__ uint32_t digit_exclusive(uint32_t v, uint32_t* s4) {
+// Exclusive prefix of v over the first WAVES waves of the block, in thread
+// order.  Every thread of the block calls it (the result is meaningful for
+// those waves only); s_w holds one word per wave.
+template <int WAVES>
+__device__ __forceinline__ uint32_t block_exclusive(uint32_t v, uint32_t* s_w) {
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t incl = wave_inclusive_sum(v);
-    if (lane == 63 && wave < 4) s4[wave] = incl;
+    if (lane == 63 && wave < (uint32_t)WAVES) s_w[wave] = incl;
     __syncthreads();
     uint32_t pre = 0;
 #pragma unroll
-    for (int w = 0; w < 4; ++w)
-        if (w < (int)wave) pre += s4[w];
+    for (int w = 0; w < WAVES; ++w)
+        if (w < (int)wave) pre += s_w[w];
     __syncthreads();
     return pre + incl - v;
 }
 
-__global__ __launch_bounds__(256) void k_sort_init(uint32_t* __restrict__ hist_tickets,
+// Exclusive prefix over the digits of a block: thread t of the first WAVES
+// waves holds the DPT values of digits t * DPT + k.
+template <int WAVES, int DPT>
+__device__ __forceinline__ void digits_exclusive(const uint32_t (&v)[DPT], uint32_t (&out)[DPT], uint32_t* s_w) {
+    uint32_t run = 0;
+#pragma unroll
+    for (int k = 0; k < DPT; ++k) {
+        out[k] = run;
+        run += v[k];
+    }
+    const uint32_t base = block_exclusive<WAVES>(run, s_w);
+#pragma unroll
+    for (int k = 0; k < DPT; ++k) out[k] += base;
+}
+
+// Zeroes the histograms and tickets of every pass, and the look-back words
+// of the first pass's tiles (the later passes clear their successors' words
+// themselves).
+__global__ __launch_bounds__(256) void k_sort_init(uint32_t* __restrict__ hist_tickets, size_t hist_words,
                                                    uint32_t* __restrict__ status, const uint32_t* n_dev,
-                                                   uint64_t n_host, uint32_t passes, uint32_t max_tiles,
-                                                   uint32_t tile_keys) {
+                                                   uint64_t n_host, uint32_t tile_keys, uint32_t radix) {
     const uint32_t n = read_n(n_dev, n_host);
     const uint32_t tiles = ceil_div(n, tile_keys);
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     const size_t t0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    for (size_t i = t0; i < (size_t)kMaxPasses * kRadix + kMaxPasses; i += stride) hist_tickets[i] = 0;
-    const size_t per_pass = (size_t)tiles * kRadix;
-    for (uint32_t p = 0; p < passes; ++p) {
-        uint32_t* s = status + (size_t)p * max_tiles * kRadix;
-        for (size_t i = t0; i < per_pass; i += stride) s[i] = 0;
-    }
+    for (size_t i = t0; i < hist_words; i += stride) hist_tickets[i] = 0;
+    const size_t words = (size_t)tiles * radix;
+    for (size_t i = t0; i < words; i += stride) status[i] = 0;
 }
 
 // 1024-thread blocks (one per CU) take 1024 x kHistItems-key tiles in a
 // grid-stride loop; each thread issues all kHistItems loads of a tile before
-// counting, so the loop is not bound by one load's latency per key.  Waves
-// w and w + 4k share LDS copy w % 4 of the bins.
-template <typename K>
+// counting, so the loop is not bound by one load's latency per key.  8-bit
+// digits: waves w and w + 4k share LDS copy w % 4 of the bins; wider digits
+// spread over more bins and use one copy.
+template <typename K, int RB>
 __global__ __launch_bounds__(1024) void k_sort_hist(const K* __restrict__ keys, const uint32_t* n_dev,
                                                    uint64_t n_host, uint32_t begin_bit, uint32_t end_bit,
                                                    uint32_t passes, uint32_t* __restrict__ hist) {
-    __shared__ uint32_t s_hist[4][kMaxPasses * kRadix];
+    constexpr int R = 1 << RB;
+    constexpr int COPIES = RB <= 8 ? 4 : 1;
+    constexpr int MAXP = (int)(sizeof(K) * 8 + RB - 1) / RB;
+    __shared__ uint32_t s_hist[COPIES][MAXP * R];
     const uint32_t n = read_n(n_dev, n_host);
-    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, copy = wave & 3;
-    for (uint32_t w = 0; w < 4; ++w)
-        for (uint32_t i = threadIdx.x; i < passes * kRadix; i += blockDim.x) s_hist[w][i] = 0;
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, copy = wave % COPIES;
+    for (uint32_t w = 0; w < COPIES; ++w)
+        for (uint32_t i = threadIdx.x; i < passes * R; i += blockDim.x) s_hist[w][i] = 0;
     __syncthreads();
     constexpr uint32_t kBlockKeys = 1024 * kHistItems;
     for (uint32_t t = blockIdx.x; (uint64_t)t * kBlockKeys < n; t += gridDim.x) {
@@ -105,8 +157,8 @@ __global__ __launch_bounds__(1024) void k_sort_hist(const K* __restrict__ keys, 
             const uint32_t valid = min(64u, n - p0);  // lanes [0, valid) hold keys
             const bool ok = lane < valid;
             for (uint32_t p = 0; p < passes; ++p) {
-                const uint32_t shift = begin_bit + p * kRadixBits;
-                const uint32_t bits = min((uint32_t)kRadixBits, end_bit - shift);
+                const uint32_t shift = begin_bit + p * RB;
+                const uint32_t bits = min((uint32_t)RB, end_bit - shift);
                 const uint32_t d = (uint32_t)(k[i] >> shift) & ((1u << bits) - 1u);
                 // previous lane's digit (wave_shr:1); lane 0 always starts a run
                 const uint32_t prev = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)d, 0x138, 0xF, 0xF, false);
@@ -115,48 +167,67 @@ __global__ __launch_bounds__(1024) void k_sort_hist(const K* __restrict__ keys, 
                 if (head) {
                     const uint64_t after = lane == 63 ? 0ull : heads & ~((2ull << lane) - 1ull);
                     const uint32_t next = after ? (uint32_t)__builtin_ctzll(after) : valid;
-                    atomicAdd(&s_hist[copy][p * kRadix + d], next - lane);
+                    atomicAdd(&s_hist[copy][p * R + d], next - lane);
                 }
             }
         }
     }
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < passes * kRadix; i += blockDim.x) {
-        const uint32_t v = s_hist[0][i] + s_hist[1][i] + s_hist[2][i] + s_hist[3][i];
+    for (uint32_t i = threadIdx.x; i < passes * R; i += blockDim.x) {
+        uint32_t v = 0;
+#pragma unroll
+        for (int w = 0; w < COPIES; ++w) v += s_hist[w][i];
         if (v) atomicAdd(&hist[i], v);
     }
 }
 
-template <typename K, int THREADS>
+// One pass.  status: this pass's look-back words (tile * R + digit), zero
+// for every tile of the key count; clear_next: the next pass's words, which
+// this pass zeroes (null on the last pass).
+template <typename K, int THREADS, int RB>
 __global__ __launch_bounds__(THREADS) void k_onesweep(
     const K* __restrict__ kin, const uint32_t* __restrict__ vin, K* __restrict__ kout,
     uint32_t* __restrict__ vout, const uint32_t* n_dev, uint64_t n_host, uint32_t shift, uint32_t bits,
-    const uint32_t* __restrict__ hist, uint32_t* ticket, uint32_t* status, uint32_t* error) {
-    using G = SortGeom<K, THREADS>;
-    constexpr int WAVES = G::kWaves, ITEMS = G::kItems, TILE = G::kTile;
-    __shared__ uint32_t s_wave_cnt[WAVES][kRadix];
-    __shared__ uint32_t s_global[kRadix];
-    __shared__ uint32_t s_local[kRadix];
-    __shared__ uint32_t s_scatter[kRadix];
+    const uint32_t* __restrict__ hist, uint32_t* ticket, uint32_t* status, uint32_t* clear_next,
+    uint32_t* error) {
+    using G = SortGeom<K, THREADS, RB>;
+    constexpr int WAVES = G::kWaves, ITEMS = G::kItems, TILE = G::kTile, R = G::kRadixN, DPT = G::kDPT,
+                  LB = G::kLB;
+    using CntT = typename G::CntT;
+    __shared__ __align__(16) CntT s_wave_cnt[WAVES][R];
+    __shared__ uint32_t s_global[R];
+    __shared__ uint32_t s_local[R];
+    __shared__ uint32_t s_scatter[R];
     __shared__ K s_keys[TILE];
     __shared__ uint32_t s_vals[TILE];
-    __shared__ uint32_t s4[4];
+    __shared__ uint32_t s_w[WAVES];
     __shared__ uint32_t s_tile;
 
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const bool digit_thread = tid < (uint32_t)kRadix;  // threads 0..255 own one digit each
+    const bool digit_thread = tid < (uint32_t)G::kDigitThreads;
+    const uint32_t d0 = tid * DPT;  // first digit of this thread
     const uint32_t n = read_n(n_dev, n_host);
     const uint32_t num_tiles = ceil_div(n, (uint32_t)TILE);
     const uint32_t mask = (1u << bits) - 1u;
+    if (clear_next) {
+        const size_t words = (size_t)num_tiles * R, stride = (size_t)gridDim.x * THREADS;
+        for (size_t i = (size_t)blockIdx.x * THREADS + tid; i < words; i += stride) clear_next[i] = 0;
+    }
     if (blockIdx.x >= num_tiles) return;  // uniform per block
     if (tid == 0) s_tile = atomicAdd(ticket, 1u);
     {
-        const uint32_t g = digit_exclusive(digit_thread ? hist[tid] : 0u, s4);
-        if (digit_thread) s_global[tid] = g;
+        uint32_t h[DPT], g[DPT];
+#pragma unroll
+        for (int k = 0; k < DPT; ++k) h[k] = digit_thread ? hist[d0 + k] : 0u;
+        digits_exclusive<G::kDigitWaves, DPT>(h, g, s_w);
+        if (digit_thread)
+#pragma unroll
+            for (int k = 0; k < DPT; ++k) s_global[d0 + k] = g[k];
     }
 
     for (;;) {
-        for (int i = tid; i < WAVES * kRadix; i += THREADS) (&s_wave_cnt[0][0])[i] = 0;
+        uint32_t* zc = reinterpret_cast<uint32_t*>(&s_wave_cnt[0][0]);  // cleared as 32-bit words
+        for (int i = tid; i < (int)(WAVES * R * sizeof(CntT) / 4); i += THREADS) zc[i] = 0;
         __syncthreads();
         const uint32_t tile = s_tile;
         if (tile >= num_tiles) break;
@@ -187,58 +258,75 @@ __global__ __launch_bounds__(THREADS) void k_onesweep(
             uint32_t prior = 0;
             if (ok) prior = s_wave_cnt[wave][d];
             rank[i] = prior + below;
-            if (ok && below == 0) s_wave_cnt[wave][d] = prior + (uint32_t)__popcll(m);
+            if (ok && below == 0) s_wave_cnt[wave][d] = (CntT)(prior + (uint32_t)__popcll(m));
         }
         __syncthreads();
         // Per digit: wave prefixes, tile count, publication, look-back.
-        uint32_t cnt = 0;
-        uint32_t* my_status = status + (size_t)tile * kRadix + tid;
-        if (digit_thread) {
+        uint32_t cnt[DPT];
+        uint32_t* my_status = status + (size_t)tile * R + d0;
 #pragma unroll
-            for (int w = 0; w < WAVES; ++w) {
-                const uint32_t c = s_wave_cnt[w][tid];
-                s_wave_cnt[w][tid] = cnt;
-                cnt += c;
+        for (int k = 0; k < DPT; ++k) {
+            cnt[k] = 0;
+            if (digit_thread) {
+#pragma unroll
+                for (int w = 0; w < WAVES; ++w) {
+                    const uint32_t c = s_wave_cnt[w][d0 + k];
+                    s_wave_cnt[w][d0 + k] = (CntT)cnt[k];
+                    cnt[k] += c;
+                }
+                st_release_u32(my_status + k, (tile == 0 ? kInc : kAgg) | cnt[k]);
             }
-            st_release_u32(my_status, (tile == 0 ? kInc : kAgg) | cnt);
         }
-        const uint32_t local = digit_exclusive(cnt, s4);
-        uint32_t excl = 0;
+        uint32_t local[DPT];
+        digits_exclusive<G::kDigitWaves, DPT>(cnt, local, s_w);
+        uint32_t excl[DPT];
+#pragma unroll
+        for (int k = 0; k < DPT; ++k) excl[k] = 0;
         if (digit_thread && tile > 0) {
-            // Batched look-back: the words of up to kLookBack predecessors are
-            // loaded together (independent loads, one round trip), then
-            // consumed nearest first; only a word still not ready is re-polled.
+            // Batched look-back: the words of up to LB predecessors (all of
+            // this thread's digits) are loaded together (independent loads,
+            // one round trip), then consumed nearest first; only a word still
+            // not ready is re-polled.
             int64_t j = (int64_t)tile - 1;
             uint32_t spins = 0;
-            bool found = false;
-            while (!found && j >= 0) {
-                uint32_t w[kLookBack];
+            uint32_t open = (1u << DPT) - 1u;  // digits still without an inclusive prefix
+            while (open != 0 && j >= 0) {
+                uint32_t w[LB][DPT];
 #pragma unroll
-                for (int d = 0; d < kLookBack; ++d)
-                    w[d] = (j - d >= 0) ? ld_acquire_u32(status + (size_t)(j - d) * kRadix + tid) : kInc;
+                for (int t = 0; t < LB; ++t)
 #pragma unroll
-                for (int d = 0; d < kLookBack; ++d) {
-                    if (found || j - d < 0) continue;
-                    uint32_t s = w[d];
-                    while ((s >> 30) == 0) {
-                        if (++spins > kSpinLimit) {
-                            atomicOr(error, 2u);
-                            s = kInc;
-                            break;
+                    for (int k = 0; k < DPT; ++k)
+                        w[t][k] = (j - t >= 0) ? ld_acquire_u32(status + (size_t)(j - t) * R + d0 + k) : kInc;
+#pragma unroll
+                for (int t = 0; t < LB; ++t)
+#pragma unroll
+                    for (int k = 0; k < DPT; ++k) {
+                        if (!((open >> k) & 1u) || j - t < 0) continue;
+                        uint32_t s = w[t][k];
+                        while ((s >> 30) == 0) {
+                            if (++spins > kSpinLimit) {
+                                atomicOr(error, 2u);
+                                s = kInc;
+                                break;
+                            }
+                            __builtin_amdgcn_s_sleep(1);
+                            s = ld_acquire_u32(status + (size_t)(j - t) * R + d0 + k);
                         }
-                        __builtin_amdgcn_s_sleep(1);
-                        s = ld_acquire_u32(status + (size_t)(j - d) * kRadix + tid);
+                        excl[k] += s & kStatusValueMask;
+                        if (s & kInc) open &= ~(1u << k);
                     }
-                    excl += s & kStatusValueMask;
-                    if (s & kInc) found = true;
-                }
-                j -= kLookBack;
+                j -= LB;
             }
-            st_release_u32(my_status, kInc | ((excl + cnt) & kStatusValueMask));
+#pragma unroll
+            for (int k = 0; k < DPT; ++k)
+                st_release_u32(my_status + k, kInc | ((excl[k] + cnt[k]) & kStatusValueMask));
         }
         if (digit_thread) {
-            s_local[tid] = local;
-            s_scatter[tid] = s_global[tid] + excl - local;
+#pragma unroll
+            for (int k = 0; k < DPT; ++k) {
+                s_local[d0 + k] = local[k];
+                s_scatter[d0 + k] = s_global[d0 + k] + excl[k] - local[k];
+            }
         }
         __syncthreads();
         // Digit-sorted staging of the tile in LDS.
@@ -272,79 +360,119 @@ __global__ __launch_bounds__(THREADS) void k_onesweep(
     }
 }
 
-template <typename K, int THREADS>
+template <typename K, int THREADS, int RB>
 int sort_impl(Context* c, K* k0, uint32_t* v0, K* k1, uint32_t* v1, const uint32_t* n_dev, uint64_t n_host,
               uint64_t capacity, uint32_t begin_bit, uint32_t end_bit, FrameCounters* counters,
               int* result_index) {
-    using G = SortGeom<K, THREADS>;
-    const uint32_t passes = end_bit > begin_bit ? ceil_div(end_bit - begin_bit, kRadixBits) : 0;
+    using G = SortGeom<K, THREADS, RB>;
+    constexpr uint32_t R = G::kRadixN;
+    const uint32_t passes = sort_passes(begin_bit, end_bit, RB);
     *result_index = (int)(passes & 1u);
     if (passes == 0) return GSR_OK;
     if (passes > (uint32_t)kMaxPasses) return fail(c, GSR_ERR_ARGUMENT, "too many radix passes (%u)", passes);
     const uint32_t max_tiles = std::max<uint32_t>(1, ceil_div(capacity, (uint64_t)G::kTile));
-    uint32_t* hist = (uint32_t*)scratch(c, "sort_hist", ((size_t)kMaxPasses * kRadix + kMaxPasses) * 4);
-    uint32_t* status = (uint32_t*)scratch(c, "sort_status", (size_t)passes * max_tiles * kRadix * 4);
+    const size_t hist_words = (size_t)passes * R + kMaxPasses;
+    uint32_t* hist = (uint32_t*)scratch(c, "sort_hist", hist_words * 4);
+    // two sets of look-back words, alternating between passes
+    uint32_t* status = (uint32_t*)scratch(c, "sort_status", (size_t)2 * max_tiles * R * 4);
     if (!hist || !status) return GSR_ERR_ALLOC;
-    uint32_t* tickets = hist + kMaxPasses * kRadix;
+    uint32_t* tickets = hist + (size_t)passes * R;
     const uint32_t fill_blocks = std::min<uint32_t>(std::max<uint32_t>(1, max_tiles), 1024);
-    hipLaunchKernelGGL(k_sort_init, dim3(fill_blocks), dim3(256), 0, c->stream, hist, status, n_dev, n_host,
-                       passes, max_tiles, (uint32_t)G::kTile);
+    hipLaunchKernelGGL(k_sort_init, dim3(fill_blocks), dim3(256), 0, c->stream, hist, hist_words, status, n_dev,
+                       n_host, (uint32_t)G::kTile, R);
     // One block per CU: every block adds its bins to the global histogram with
     // one atomic each, and those atomics serialize per bin across blocks, so
     // fewer, larger blocks.
     const uint32_t hist_blocks = std::min<uint32_t>(
         std::max<uint32_t>(1, ceil_div(capacity, (uint64_t)1024 * kHistItems)), (uint32_t)c->cu_count);
-    hipLaunchKernelGGL(k_sort_hist<K>, dim3(hist_blocks), dim3(1024), 0, c->stream, k0, n_dev, n_host,
+    hipLaunchKernelGGL((k_sort_hist<K, RB>), dim3(hist_blocks), dim3(1024), 0, c->stream, k0, n_dev, n_host,
                        begin_bit, end_bit, passes, hist);
     const uint32_t sweep_blocks = std::min<uint32_t>(max_tiles, (uint32_t)c->cu_count * (1024 / THREADS));
     K* kb[2] = {k0, k1};
     uint32_t* vb[2] = {v0, v1};
+    uint32_t* st[2] = {status, status + (size_t)max_tiles * R};
     for (uint32_t p = 0; p < passes; ++p) {
-        const uint32_t shift = begin_bit + p * kRadixBits;
-        const uint32_t bits = std::min<uint32_t>(kRadixBits, end_bit - shift);
-        hipLaunchKernelGGL((k_onesweep<K, THREADS>), dim3(sweep_blocks), dim3(THREADS), 0, c->stream, kb[p & 1],
-                           vb[p & 1], kb[(p + 1) & 1], vb[(p + 1) & 1], n_dev, n_host, shift, bits,
-                           hist + p * kRadix, tickets + p, status + (size_t)p * max_tiles * kRadix,
-                           &counters->error);
+        const uint32_t shift = begin_bit + p * RB;
+        const uint32_t bits = std::min<uint32_t>(RB, end_bit - shift);
+        hipLaunchKernelGGL((k_onesweep<K, THREADS, RB>), dim3(sweep_blocks), dim3(THREADS), 0, c->stream, kb[p & 1],
+                           vb[p & 1], kb[(p + 1) & 1], vb[(p + 1) & 1], n_dev, n_host, shift, bits, hist + p * R,
+                           tickets + p, st[p & 1], p + 1 < passes ? st[(p + 1) & 1] : nullptr, &counters->error);
     }
     GSR_HIP(c, hipGetLastError());
     return GSR_OK;
 }
 
 // Onesweep blocks per CU: 1, 2 or 4 (1024-, 512- or 256-thread blocks).
-template <typename K>
+template <typename K, int RB>
 int sort_geometry(Context* c, K* k0, uint32_t* v0, K* k1, uint32_t* v1, const uint32_t* n_dev, uint64_t n_host,
                   uint64_t capacity, uint32_t begin_bit, uint32_t end_bit, uint32_t blocks_per_cu,
                   FrameCounters* counters, int* result_index) {
     switch (blocks_per_cu) {
         case 1:
-            return sort_impl<K, 1024>(c, k0, v0, k1, v1, n_dev, n_host, capacity, begin_bit, end_bit, counters,
-                                      result_index);
+            return sort_impl<K, 1024, RB>(c, k0, v0, k1, v1, n_dev, n_host, capacity, begin_bit, end_bit, counters,
+                                          result_index);
         case 2:
-            return sort_impl<K, 512>(c, k0, v0, k1, v1, n_dev, n_host, capacity, begin_bit, end_bit, counters,
-                                     result_index);
+            return sort_impl<K, 512, RB>(c, k0, v0, k1, v1, n_dev, n_host, capacity, begin_bit, end_bit, counters,
+                                         result_index);
         case 4:
-            return sort_impl<K, 256>(c, k0, v0, k1, v1, n_dev, n_host, capacity, begin_bit, end_bit, counters,
-                                     result_index);
+            return sort_impl<K, 256, RB>(c, k0, v0, k1, v1, n_dev, n_host, capacity, begin_bit, end_bit, counters,
+                                         result_index);
         default:
             return fail(c, GSR_ERR_ARGUMENT, "sort blocks per CU %u not in {1, 2, 4}", blocks_per_cu);
     }
 }
 
+template <typename K>
+int sort_digits(Context* c, K* k0, uint32_t* v0, K* k1, uint32_t* v1, const uint32_t* n_dev, uint64_t n_host,
+                uint64_t capacity, uint32_t begin_bit, uint32_t end_bit, uint32_t blocks_per_cu, uint32_t digit_bits,
+                FrameCounters* counters, int* result_index) {
+    switch (digit_bits) {
+        case 8:
+            return sort_geometry<K, 8>(c, k0, v0, k1, v1, n_dev, n_host, capacity, begin_bit, end_bit, blocks_per_cu,
+                                       counters, result_index);
+        case 9:
+            return sort_geometry<K, 9>(c, k0, v0, k1, v1, n_dev, n_host, capacity, begin_bit, end_bit, blocks_per_cu,
+                                       counters, result_index);
+        case 10:
+            return sort_geometry<K, 10>(c, k0, v0, k1, v1, n_dev, n_host, capacity, begin_bit, end_bit,
+                                        blocks_per_cu, counters, result_index);
+        case 11:
+            return sort_geometry<K, 11>(c, k0, v0, k1, v1, n_dev, n_host, capacity, begin_bit, end_bit,
+                                        blocks_per_cu, counters, result_index);
+        default:
+            return fail(c, GSR_ERR_ARGUMENT, "sort digit width %u not in 8..11", digit_bits);
+    }
+}
+
 }  // namespace
+
+uint32_t default_digit_bits(uint32_t key_bytes, uint32_t end_bit) {
+    if (key_bytes != 8) return 8;  // wider digits would halve the 32-bit key tile (LDS)
+    // 64-bit keys keep 8192-key tiles up to 10-bit digits: the narrowest
+    // width with the fewest passes
+    uint32_t best = 8;
+    for (uint32_t rb = 9; rb <= 10; ++rb)
+        if (sort_passes(0, end_bit, rb) < sort_passes(0, end_bit, best)) best = rb;
+    return best;
+}
+
+uint32_t sort_result_index(uint32_t begin_bit, uint32_t end_bit, uint32_t digit_bits) {
+    return sort_passes(begin_bit, end_bit, digit_bits) & 1u;
+}
 
 int launch_sort(Context* c, uint32_t key_bytes, void* k0, uint32_t* v0, void* k1, uint32_t* v1,
                 const uint32_t* n_dev, uint64_t n_host, uint64_t capacity, uint32_t begin_bit,
-                uint32_t end_bit, uint32_t blocks_per_cu, FrameCounters* counters, int* result_index) {
+                uint32_t end_bit, uint32_t blocks_per_cu, FrameCounters* counters, int* result_index,
+                uint32_t digit_bits) {
     if (capacity > kMaxSortKeys) return fail(c, GSR_ERR_ARGUMENT, "sort capacity %llu exceeds %llu",
                                              (unsigned long long)capacity, (unsigned long long)kMaxSortKeys);
     if (end_bit > key_bytes * 8 || begin_bit > end_bit)
         return fail(c, GSR_ERR_ARGUMENT, "bad bit range [%u, %u) for %u-byte keys", begin_bit, end_bit, key_bytes);
     if (key_bytes == 8)
-        return sort_geometry<uint64_t>(c, (uint64_t*)k0, v0, (uint64_t*)k1, v1, n_dev, n_host, capacity, begin_bit,
-                                       end_bit, blocks_per_cu, counters, result_index);
-    return sort_geometry<uint32_t>(c, (uint32_t*)k0, v0, (uint32_t*)k1, v1, n_dev, n_host, capacity, begin_bit,
-                                   end_bit, blocks_per_cu, counters, result_index);
+        return sort_digits<uint64_t>(c, (uint64_t*)k0, v0, (uint64_t*)k1, v1, n_dev, n_host, capacity, begin_bit,
+                                     end_bit, blocks_per_cu, digit_bits, counters, result_index);
+    return sort_digits<uint32_t>(c, (uint32_t*)k0, v0, (uint32_t*)k1, v1, n_dev, n_host, capacity, begin_bit,
+                                 end_bit, blocks_per_cu, digit_bits, counters, result_index);
 }
 
 }  // namespace gsr

@@ -49,7 +49,7 @@ class Options(ctypes.Structure):
     _fields_ = [("tile_size", ctypes.c_uint32), ("key_bytes", ctypes.c_uint32),
                 ("instance_capacity", ctypes.c_uint64), ("timing", ctypes.c_uint32),
                 ("save_state", ctypes.c_uint32), ("raster_waves_per_cu", ctypes.c_uint32),
-                ("sort_blocks_per_cu", ctypes.c_uint32)]
+                ("sort_blocks_per_cu", ctypes.c_uint32), ("sort_digit_bits", ctypes.c_uint32)]
 
 
 class TimingRow(ctypes.Structure):
@@ -100,6 +100,8 @@ def load():
     L.gsr_scan_offsets.argtypes = [_VP, _VP, _U32, _VP, _VP]
     L.gsr_emit.argtypes = [_VP, _VP, _VP, _U32, _U32, _U32, _U64, _U32, _VP, _VP]
     L.gsr_sort_pairs.argtypes = [_VP, _U32, _VP, _VP, _VP, _VP, _U64, _U32, _U32, ctypes.POINTER(_INT)]
+    L.gsr_sort_pairs_geometry.argtypes = [_VP, _U32, _VP, _VP, _VP, _VP, _U64, _U32, _U32, _U32, _U32,
+                                          ctypes.POINTER(_INT)]
     L.gsr_tile_ranges.argtypes = [_VP, _VP, _U32, _U64, _U32, _VP]
     L.gsr_rasterize.argtypes = [_VP, _U32, _U32, _U32, _U32, _U64, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP]
     L.gsr_backward.argtypes = [_VP, _VP, _U32, ctypes.POINTER(Camera), _VP, _VP, _VP]
@@ -121,9 +123,9 @@ def last_error(ctx=None) -> str:
 
 
 def options(tile_size=16, key_bytes=KEY_AUTO, capacity=0, timing=False, save_state=False, raster_waves_per_cu=0,
-            sort_blocks_per_cu=0) -> Options:
+            sort_blocks_per_cu=0, sort_digit_bits=0) -> Options:
     return Options(tile_size, key_bytes, capacity, 1 if timing else 0, 1 if save_state else 0, raster_waves_per_cu,
-                   sort_blocks_per_cu)
+                   sort_blocks_per_cu, sort_digit_bits)
 
 
 def _ptr(x):
@@ -276,10 +278,16 @@ class Context:
         self._check(self.L.gsr_emit(self.h, _ptr(offsets), _ptr(meta), count, width, tile, capacity, key_bytes,
                                     _ptr(keys), _ptr(vals)))
 
-    def sort_pairs(self, key_bytes, k0, v0, k1, v1, n, begin_bit, end_bit) -> int:
+    def sort_pairs(self, key_bytes, k0, v0, k1, v1, n, begin_bit, end_bit, blocks_per_cu=None,
+                   digit_bits=None) -> int:
         idx = _INT()
-        self._check(self.L.gsr_sort_pairs(self.h, key_bytes, _ptr(k0), _ptr(v0), _ptr(k1), _ptr(v1), n, begin_bit,
-                                          end_bit, ctypes.byref(idx)))
+        if blocks_per_cu is None and digit_bits is None:
+            self._check(self.L.gsr_sort_pairs(self.h, key_bytes, _ptr(k0), _ptr(v0), _ptr(k1), _ptr(v1), n,
+                                              begin_bit, end_bit, ctypes.byref(idx)))
+        else:
+            self._check(self.L.gsr_sort_pairs_geometry(self.h, key_bytes, _ptr(k0), _ptr(v0), _ptr(k1), _ptr(v1), n,
+                                                       begin_bit, end_bit, blocks_per_cu or 1, digit_bits or 8,
+                                                       ctypes.byref(idx)))
         return idx.value
 
     def tile_ranges(self, keys, key_bytes, n, num_tiles, ranges):
