@@ -123,6 +123,7 @@ def batch_leg(ctx, torch, lib, scenes, reps=10):
     ratio = (len(us) / t_batch) / (1.0 / t_plain)
     return {"views": len(us), "splats": n, "instances": instances, "tile": opts.tile_size,
             "raster_waves_per_cu": opts.raster_waves_per_cu, "sort_blocks_per_cu": opts.sort_blocks_per_cu,
+            "sort_digit_bits": opts.sort_digit_bits,
             "batch_ms": round(t_batch * 1e3, 4), "views_per_s": round(len(us) / t_batch, 2),
             "plain_mean_ms_per_view": round(t_plain * 1e3, 4), "per_view_speedup_vs_plain": round(ratio, 3),
             "autotune_best_ms": round(tuned_ms, 4)}
