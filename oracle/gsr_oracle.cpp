@@ -901,11 +901,13 @@ void backward_f64(const float* packed, uint32_t n, const Cam& cam, uint32_t tile
 }
 
 // FP32 replay of the raster adjoint with the same steps as the GPU kernel
-// (T recovered by dividing by 1 - alpha, colour-behind recurrence, per-splat
-// moments of dL/dpower turned into the 2-D gradient with the conic), per-splat
-// sums in double.  The kernel multiplies by a 1-ulp hardware reciprocal
-// instead of dividing and converts the moments in FP32; otherwise the
-// arithmetic is the same, so the twin separates FP32 term error from
+// (T recovered with 1 / (1 - alpha), the dL/dRGB-weighted light from behind
+// GS = sum_ch g_ch sum_behind c_j alpha_j T_j, so that
+// T_i (c_i - behind_i / T_{i+1}) . g = T_b (c . g) - GS / (1 - alpha),
+// per-splat moments of dL/dpower turned into the 2-D gradient with the conic),
+// per-splat sums in double.  The kernel takes a 1-ulp hardware reciprocal
+// instead of the IEEE division and converts the moments in FP32; otherwise
+// the arithmetic is the same, so the twin separates FP32 term error from
 // summation order.
 void raster_backward_f32(uint32_t W, uint32_t H, uint32_t tile, const uint32_t* ranges, const uint32_t* vals,
                          const float* conic, const float* xy, const float* rgb, const float* tfinal,
@@ -923,7 +925,7 @@ void raster_backward_f32(uint32_t W, uint32_t H, uint32_t tile, const uint32_t* 
                 float T = tfinal[o];
                 const uint32_t L = last[o];
                 const float gr = dLdC[o], gg = dLdC[(size_t)W * H + o], gb = dLdC[(size_t)2 * W * H + o];
-                float Br = 0, Bg = 0, Bb = 0;
+                float GS = 0;
                 for (uint32_t pos = L; pos >= 1; --pos) {
                     const uint32_t g = vals[start + pos - 1];
                     const float A = conic[g * 4], B = conic[g * 4 + 1], C = conic[g * 4 + 2], op = conic[g * 4 + 3];
@@ -935,10 +937,12 @@ void raster_backward_f32(uint32_t W, uint32_t H, uint32_t tile, const uint32_t* 
                     const float oe = op * e;
                     const float alpha = std::min(0.99f, oe);
                     if (alpha < 1.0f / 255.0f) continue;
-                    const float Tb = T / (1.0f - alpha);
+                    const float r = 1.0f / (1.0f - alpha);
+                    const float Tb = T * r;
                     T = Tb;
                     const float cr = rgb[g * 4], cg = rgb[g * 4 + 1], cb = rgb[g * 4 + 2];
-                    const float gal = fmaf(cr - Br, gr, fmaf(cg - Bg, gg, (cb - Bb) * gb)) * Tb;
+                    const float gc = fmaf(cr, gr, fmaf(cg, gg, cb * gb));
+                    const float gal = fmaf(Tb, gc, -(r * GS));
                     const float w = alpha * Tb;
                     double* a = &grad2d[(size_t)g * G2];
                     a[6] += w * gr; a[7] += w * gg; a[8] += w * gb;
@@ -953,10 +957,7 @@ void raster_backward_f32(uint32_t W, uint32_t H, uint32_t tile, const uint32_t* 
                         a[4] += gdy * dy;
                         a[5] += e * gal;
                     }
-                    const float om = 1.0f - alpha;
-                    Br = fmaf(alpha, cr, om * Br);
-                    Bg = fmaf(alpha, cg, om * Bg);
-                    Bb = fmaf(alpha, cb, om * Bb);
+                    GS = fmaf(w, gc, GS);
                 }
             }
     }

@@ -69,3 +69,28 @@ def test_gradient_mass_bounds_the_gradient(orc):
     assert (m2[:, :9] >= np.abs(g2[:, :9]) * (1 - 1e-12)).all()
     assert (pm >= np.abs(g) * (1 - 1e-9) - 1e-300).all()
     assert (m2[:, :9] > 0).any(1).sum() > sc.count // 2
+
+
+def test_fp32_twin_matches_fp64_adjoint(orc):
+    """The FP32 twin of the raster adjoint (the kernel's arithmetic on the CPU:
+    T recovered through 1 / (1 - alpha), the dL/dRGB-weighted light from
+    behind as one running sum) against the FP64 adjoint on the same FP32
+    forward decisions: every 2-D gradient element within 1e-3 |g64| + 1e-4 m
+    (m the summed magnitude of the element's terms)."""
+    sc, cam = _scene()
+    n = sc.count
+    u = cam.uniform(n)
+    W, H = cam.width, cam.height
+    dLdC = np.random.default_rng(6).normal(size=(3, H, W))
+    pre = orc.preprocess(sc.packed(), n, u, tile=16)
+    offs, total = orc.scan(np.concatenate([pre["tiles"], [0]]).astype(np.uint32))
+    keys, vals = orc.emit(offs, pre["meta"], n, W, 16, 64 * n)
+    ks, vs = orc.sort_stable(keys, vals)
+    ranges = orc.tile_ranges(ks, ((W + 15) // 16) * ((H + 15) // 16))
+    _, T, last = orc.raster(W, H, 16, ranges, vs, pre["conic"], pre["xy"], pre["rgb"])
+    twin = orc.raster_backward_f32(W, H, 16, ranges, vs, pre["conic"], pre["xy"], pre["rgb"], T, last,
+                                   np.asarray(dLdC, np.float32), n)
+    _, ref2, _, m2, _ = orc.backward(sc.packed(), n, u, dLdC, decisions_f32=True, want_mass=True)
+    a, b, m = twin[:, :9], ref2[:, :9], m2[:, :9]
+    assert (np.abs(a - b) <= 1e-3 * np.abs(b) + 1e-4 * m + 1e-30).all(), np.max(np.abs(a - b) / (1e-3 * np.abs(b) + 1e-4 * m + 1e-30))
+    assert np.linalg.norm(a - b) < 1e-4 * np.linalg.norm(b)

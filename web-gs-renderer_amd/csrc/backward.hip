@@ -7,8 +7,8 @@
 //   starting at its stored last contributor with its stored final
 //   transmittance; alphas are recomputed with the forward's exact FP32 steps
 //   (so the blend decisions are the forward's), T is recovered as
-//   T * rcp(1 - alpha), and the colour behind each splat is accumulated on the
-//   way.  Per splat the 9 gradient terms are summed over the wave (8 by a
+//   T * rcp(1 - alpha), and the light arriving from behind each splat is
+//   accumulated on the way (as one dL/dRGB-weighted sum per pixel).  Per splat the 9 gradient terms are summed over the wave (8 by a
 //   swap-and-add transposition, 1 by DPP) and added to the splat's 2-D
 //   gradient with float atomics, one instruction for the 8.
 // k_preprocess_backward: one thread per splat, turns the raster's sums into
@@ -122,7 +122,11 @@ __device__ __forceinline__ void backward_block(uint32_t W, uint32_t H, uint32_t 
     const uint32_t x = x0 + lane % 8, y = y0 + lane / 8;
     const float fx = (float)x, fy = (float)y;
     const size_t plane = (size_t)W * H;
-    float T = 1.0f, gr = 0.0f, gg = 0.0f, gb = 0.0f, Br = 0.0f, Bg = 0.0f, Bb = 0.0f;
+    // GS: sum over channels of dL/dRGB times the colour the splats behind the
+    // current one contribute (unnormalised: each c_j alpha_j T_j), so that
+    // dL/dalpha-type terms T_i (c_i - behind_i / T_{i+1}) . g collapse to
+    // T_b (c . g) - GS / (1 - alpha)
+    float T = 1.0f, gr = 0.0f, gg = 0.0f, gb = 0.0f, GS = 0.0f;
     uint32_t lst = 0;
     if (x < W && y < H) {
         const size_t o = (size_t)y * W + x;
@@ -190,8 +194,10 @@ __device__ __forceinline__ void backward_block(uint32_t W, uint32_t H, uint32_t 
             // Undo the blend (v_rcp_f32: 1 ulp, no IEEE division sequence);
             // formed before the skip test so that the colour is read from
             // LDS together with the rest of the record.
-            const float Tb = T * __builtin_amdgcn_rcpf(1.0f - alpha);
-            const float gal = __builtin_fmaf(cr - Br, gr, __builtin_fmaf(cg - Bg, gg, (cb - Bb) * gb)) * Tb;
+            const float r = __builtin_amdgcn_rcpf(1.0f - alpha);
+            const float Tb = T * r;
+            const float gc = __builtin_fmaf(cr, gr, __builtin_fmaf(cg, gg, cb * gb));
+            const float gal = __builtin_fmaf(Tb, gc, -(r * GS));
             asm volatile("" ::"v"(gal));  // keeps gal (and the colour read) ahead of the branch
             if (ballot(hit) == 0) continue;
             const bool grad_ok = hit & (oe <= kAlphaClamp);
@@ -200,19 +206,17 @@ __device__ __forceinline__ void backward_block(uint32_t W, uint32_t H, uint32_t 
             const float gdx = gpow * dx, gdy = gpow * dy;
             const float a[8] = {gdx, gdy, gdx * dx, gdx * dy, gdy * dy, grad_ok ? e * gal : 0.0f, w * gr, w * gg};
             const float a_b = w * gb;
-            // state before this splat where it was blended (selects, no branch)
-            const float om = 1.0f - alpha;
+            // state before this splat where it was blended (w = 0 elsewhere)
             T = hit ? Tb : T;
-            Br = hit ? __builtin_fmaf(alpha, cr, om * Br) : Br;
-            Bg = hit ? __builtin_fmaf(alpha, cg, om * Bg) : Bg;
-            Bb = hit ? __builtin_fmaf(alpha, cb, om * Bb) : Bb;
+            GS = __builtin_fmaf(w, gc, GS);
             float s8, sb;
             fold9(a, a_b, s8, sb);
             const uint32_t g = __float_as_uint(s.cl.w);
             if (g < splats) {
                 float* dst = grad2d + (size_t)g * 12;
-                if ((lane & 7) == 0) unsafeAtomicAdd(dst + tag, s8);
-                if (lane == 63) unsafeAtomicAdd(dst + 8, sb);
+                // one instruction: lanes 0, 8, .., 56 add the folded 8, lane 63 the 9th
+                const bool b8 = lane == 63;
+                if ((lane & 7) == 0 || b8) unsafeAtomicAdd(dst + (b8 ? 8u : tag), b8 ? sb : s8);
             }
         }
     }
