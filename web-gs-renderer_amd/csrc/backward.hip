@@ -239,10 +239,10 @@ __global__ __launch_bounds__(256) void k_raster_backward(
                    rec, tfinal, last, dLdC, grad2d);
 }
 
-// View batches: the tiles of every view, heaviest first (order[] over the
-// global tile ids); views[v] = {W, H, tiles_x, tile_base}, pix[v] the float
-// offset of view v's [3][H][W] image (and pix[v] / 3 its pixel offset).
-// Entries (vals) are v * n + g.
+// View batches: the tiles of every view, view by view and heaviest first
+// within a view (order[] over the global tile ids); views[v] = {W, H,
+// tiles_x, tile_base}, pix[v] the float offset of view v's [3][H][W] image
+// (and pix[v] / 3 its pixel offset).  Entries (vals) are v * n + g.
 template <int TS>
 __global__ __launch_bounds__(256) void k_raster_backward_batch(
     const uint4* __restrict__ views, const uint64_t* __restrict__ pix, uint32_t num_views, uint32_t total_tiles,
@@ -730,7 +730,9 @@ int launch_backward_batch(Context* c, const float* params, uint32_t count, uint3
     const uint32_t* last = (const uint32_t*)buffer(c, "b_last", 0);
     const FrameCounters* counters = (const FrameCounters*)buffer(c, "b_counters", 0);
     mark(c, "backward raster");
-    if ((rc = launch_tile_order(c, ranges, total_tiles, order))) return rc;
+    // view by view (heaviest first within a view): the atomics of a stretch of
+    // the pass then land in one view's 2-D gradient sums, not all views'
+    if ((rc = launch_tile_order(c, ranges, total_tiles, order, info, views))) return rc;
     const uint32_t blocks = ceil_div(total_tiles * (tile * tile / 64), 4u);
     switch (tile) {
         case 8:

@@ -249,8 +249,11 @@ int launch_raster(Context* c, uint32_t W, uint32_t H, uint32_t tile, uint32_t sp
                   uint32_t n_host, const uint32_t* ranges, const uint32_t* vals, const SplatRec* rec, float* out,
                   float* tfinal, uint32_t* last, const uint32_t* order = nullptr);
 int launch_fill_u32(Context* c, uint32_t* p, uint32_t value, size_t count);
-// order[0..tiles) = the tiles, heaviest (longest instance range) first.
-int launch_tile_order(Context* c, const uint2* ranges, uint32_t tiles, uint32_t* order);
+// order[0..tiles) = the tiles, heaviest (longest instance range) first; with
+// views (device table, views[v].w = first tile of view v) heaviest first
+// within each view, views in turn.
+int launch_tile_order(Context* c, const uint2* ranges, uint32_t tiles, uint32_t* order,
+                      const uint4* views = nullptr, uint32_t num_views = 0);
 // Zeroes up to 4 regions (pointer, u32 count) with one launch.
 struct FillRegion {
     uint32_t* p;

@@ -183,11 +183,20 @@ __device__ __forceinline__ void raster_block(uint32_t W, uint32_t H, uint32_t x0
 // here, tiles bucketed by instance count on a log scale (4 classes per
 // octave) and the classes in descending order, so the longest tiles start
 // first and the short ones fill in behind them instead of leaving a tail.
-// One workgroup: its LDS holds the 128 class counters.
+// One workgroup per range of tiles (its LDS holds the 128 class counters):
+// with views (nullable; views[v].w = first tile of view v) workgroup v orders
+// view v's tiles within their own slots, so the order stays view by view and
+// a pass over it touches one view's per-splat data at a time.
 __global__ __launch_bounds__(1024) void k_tile_order(const uint2* __restrict__ ranges, uint32_t tiles,
+                                                     const uint4* __restrict__ views, uint32_t num_views,
                                                      uint32_t* __restrict__ order) {
     __shared__ uint32_t s_cnt[128];
     const uint32_t tid = threadIdx.x;
+    uint32_t t0 = 0, t1 = tiles;
+    if (views) {
+        t0 = views[blockIdx.x].w;
+        t1 = blockIdx.x + 1 < num_views ? views[blockIdx.x + 1].w : tiles;
+    }
     if (tid < 128) s_cnt[tid] = 0;
     __syncthreads();
     auto cls = [](uint32_t c) -> uint32_t {
@@ -195,7 +204,7 @@ __global__ __launch_bounds__(1024) void k_tile_order(const uint2* __restrict__ r
         const uint32_t lg = 31 - __builtin_clz(c);
         return min(127u, 4 * lg + ((c >> (lg - 2)) & 3u) - 4);
     };
-    for (uint32_t t = tid; t < tiles; t += 1024) {
+    for (uint32_t t = t0 + tid; t < t1; t += 1024) {
         const uint2 r = ranges[t];
         atomicAdd(&s_cnt[127 - cls(r.y - r.x)], 1u);
     }
@@ -207,9 +216,9 @@ __global__ __launch_bounds__(1024) void k_tile_order(const uint2* __restrict__ r
         s_cnt[2 * tid + 1] = incl - b;
     }
     __syncthreads();
-    for (uint32_t t = tid; t < tiles; t += 1024) {
+    for (uint32_t t = t0 + tid; t < t1; t += 1024) {
         const uint2 r = ranges[t];
-        order[atomicAdd(&s_cnt[127 - cls(r.y - r.x)], 1u)] = t;
+        order[t0 + atomicAdd(&s_cnt[127 - cls(r.y - r.x)], 1u)] = t;
     }
 }
 
@@ -294,9 +303,11 @@ int launch_fill_u32(Context* c, uint32_t* p, uint32_t value, size_t count) {
     return GSR_OK;
 }
 
-int launch_tile_order(Context* c, const uint2* ranges, uint32_t tiles, uint32_t* order) {
+int launch_tile_order(Context* c, const uint2* ranges, uint32_t tiles, uint32_t* order, const uint4* views,
+                      uint32_t num_views) {
     if (tiles == 0) return GSR_OK;
-    hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(1024), 0, c->stream, ranges, tiles, order);
+    hipLaunchKernelGGL(k_tile_order, dim3(views ? num_views : 1), dim3(1024), 0, c->stream, ranges, tiles, views,
+                       num_views, order);
     GSR_HIP(c, hipGetLastError());
     return GSR_OK;
 }
