@@ -152,3 +152,31 @@ def test_rgba8_conversion(torch_cuda, ctx):
     assert np.array_equal(out[..., :3], ref) and (out[..., 3] == 255).all()
     flipped = ctx.to_rgba8(d, W, H, flip_y=True)
     assert np.array_equal(flipped, out[::-1])
+
+
+def test_empty_scene_renders_black(torch_cuda, ctx):
+    """Zero splats: a black frame through every entry point (the reference
+    renders an empty scene as background), T_final 1 and no last contributor
+    for the backward, which has no gradients to produce."""
+    cam = scenes.Camera((0.0, 0.0, -4.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 60.0, 200, 120)
+    u = cam.uniform(0)
+    ctx.scene_upload("empty", np.zeros(0, np.float32), 0)
+    ctx.scene_select("empty")
+    img = ctx.render(u, lib.options())
+    assert img.shape == (3, 120, 200) and not img.any()
+    assert ctx.instance_count() == 0
+    out = torch_cuda.full((3 * 120 * 200,), 7.0, device="cuda")
+    torch_cuda.cuda.synchronize()
+    ctx.forward(None, 0, u, lib.options(save_state=True), out)
+    ctx.backward(None, 0, u, torch_cuda.zeros(3 * 120 * 200, device="cuda"), None)
+    ctx.synchronize()
+    assert not out.any().item()
+    us = [u, scenes.Camera((1.0, 0.0, -4.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 50.0, 64, 48).uniform(0)]
+    offs = ctx.batch_layout(0, us)
+    bout = torch_cuda.full((offs[-1],), 7.0, device="cuda")
+    torch_cuda.cuda.synchronize()
+    ctx.render_batch(None, 0, us, lib.options(save_state=True), bout)
+    ctx.backward_batch(None, 0, us, torch_cuda.zeros(offs[-1], device="cuda"), None)
+    ctx.synchronize()
+    assert not bout.any().item() and ctx.batch_instance_count() == 0
+    ctx.scene_free("empty")

@@ -669,6 +669,7 @@ __global__ __launch_bounds__(256) void k_preprocess_backward_batch(const float* 
 
 int launch_backward(Context* c, const float* params, uint32_t count, const CamK& cam, const float* dLdC,
                     float* grad, float* grad2d_out) {
+    if (count == 0) return GSR_OK;  // no splats, no gradients
     const uint32_t W = c->state_w, H = c->state_h, tile = c->state_tile;
     const uint32_t tx = ceil_div(W, tile), ty = ceil_div(H, tile);
     const size_t N = count;
@@ -717,6 +718,7 @@ int launch_backward(Context* c, const float* params, uint32_t count, const CamK&
 int launch_backward_batch(Context* c, const float* params, uint32_t count, uint32_t views, uint32_t tile,
                           uint32_t total_tiles, const CamK* cams, const uint4* info, const uint64_t* pix,
                           int sorted_index, const float* dLdC, float* grad, float* grad2d_out) {
+    if (count == 0) return GSR_OK;  // no splats, no gradients
     const size_t NV = (size_t)count * views;
     float* g2 = (float*)buffer(c, "b_grad2d", NV * 48);
     uint32_t* order = (uint32_t*)buffer(c, "b_tile_order", (size_t)total_tiles * 4);
@@ -772,7 +774,7 @@ int gsr_backward(gsr_context* ctx, const float* params_dev, uint32_t count, cons
                  const float* dL_drgb_dev, float* grad_params_dev, float* grad2d_dev) {
     Context* c = reinterpret_cast<Context*>(ctx);
     if (!c) return fail(nullptr, GSR_ERR_ARGUMENT, "null context");
-    if (!params_dev || !cam || !dL_drgb_dev || !grad_params_dev)
+    if (!cam || ((!params_dev || !dL_drgb_dev || !grad_params_dev) && count > 0))
         return fail(c, GSR_ERR_ARGUMENT, "gsr_backward: null argument");
     if (!c->have_state || c->state_count != count || c->state_w != (uint32_t)cam->texture_size[0] ||
         c->state_h != (uint32_t)cam->texture_size[1])

@@ -143,6 +143,14 @@ static int enqueue_batch(Context* c, const float* params, uint32_t count, const 
         last = (uint32_t*)buffer(c, "b_last", p.floats / 3 * 4);
         if (!tfinal || !last) return GSR_ERR_ALLOC;
     }
+    if (count == 0) {  // no splats: black views (T_final 1, no last contributor)
+        GSR_HIP(c, hipMemcpyAsync(tables, c->staging, tbytes, hipMemcpyHostToDevice, c->stream));
+        int rc = launch_fill_u32(c, (uint32_t*)counters, 0u, sizeof(FrameCounters) / 4);
+        if (!rc) rc = launch_fill_u32(c, (uint32_t*)out, 0u, p.floats);
+        if (!rc && p.save) rc = launch_fill_u32(c, (uint32_t*)tfinal, 0x3F800000u, p.floats / 3);
+        if (!rc && p.save) rc = launch_fill_u32(c, last, 0u, p.floats / 3);
+        return rc;
+    }
     c->scratch_prefix = "b_";
     struct Restore {
         Context* c;
@@ -314,7 +322,7 @@ int gsr_render_batch(gsr_context* ctx, const float* params_dev, uint32_t count, 
                      const gsr_options* opts, int use_graph, float* out_rgb_dev, uint64_t* out_offsets) {
     Context* c = reinterpret_cast<Context*>(ctx);
     if (!c) return fail(nullptr, GSR_ERR_ARGUMENT, "null context");
-    if (!params_dev || count == 0) return fail(c, GSR_ERR_ARGUMENT, "gsr_render_batch: no parameters");
+    if (!params_dev && count > 0) return fail(c, GSR_ERR_ARGUMENT, "gsr_render_batch: no parameters");
     hipSetDevice(c->device);
     c->timing_on = opts && opts->timing;
     return run_batch(c, params_dev, count, cams, views, opts, use_graph, out_rgb_dev, out_offsets);
@@ -392,7 +400,7 @@ int gsr_backward_batch(gsr_context* ctx, const float* params_dev, uint32_t count
                        uint32_t views, const float* dL_drgb_dev, float* grad_params_dev, float* grad2d_dev) {
     Context* c = reinterpret_cast<Context*>(ctx);
     if (!c) return fail(nullptr, GSR_ERR_ARGUMENT, "null context");
-    if (!params_dev || !cams || !dL_drgb_dev || !grad_params_dev)
+    if (!cams || ((!params_dev || !dL_drgb_dev || !grad_params_dev) && count > 0))
         return fail(c, GSR_ERR_ARGUMENT, "gsr_backward_batch: null argument");
     const BatchState& s = c->bstate;
     if (!s.valid || s.count != count || s.views != views)

@@ -212,9 +212,15 @@ void* gsr_stream(gsr_context* ctx) {
 int gsr_scene_upload(gsr_context* ctx, const char* name, const float* packed, uint32_t count) {
     Context* c = reinterpret_cast<Context*>(ctx);
     if (!c) return fail(nullptr, GSR_ERR_ARGUMENT, "null context");
-    if (!name || !packed) return fail(c, GSR_ERR_ARGUMENT, "gsr_scene_upload: null name or data");
-    if (count == 0) return fail(c, GSR_ERR_ARGUMENT, "gsr_scene_upload: zero splats");
+    if (!name || (!packed && count > 0)) return fail(c, GSR_ERR_ARGUMENT, "gsr_scene_upload: null name or data");
     hipSetDevice(c->device);
+    if (count == 0) {  // an empty scene renders black frames
+        auto it = c->scenes.find(name);
+        if (it != c->scenes.end()) hipFree(it->second.params);
+        c->scenes[name] = Scene{nullptr, 0};
+        if (c->current.empty()) c->current = name;
+        return GSR_OK;
+    }
     const size_t bytes = (size_t)count * kParams * sizeof(float);
     float* planar = nullptr;
     float* staging = nullptr;

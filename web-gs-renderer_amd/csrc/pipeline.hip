@@ -80,10 +80,32 @@ int enqueue_forward(Context* c, const float* params, uint32_t count, const gsr_c
     if (!tiles || !offsets || !meta || !rec || !order || !k0 || !k1 || !v0 || !v1 || !ranges || !image ||
         (save_state && (!tfinal || !last)) || !counters)
         return GSR_ERR_ALLOC;
+    int rc;
+    if (count == 0) {
+        // no splats: a black frame (T_final 1, no last contributor)
+        if ((rc = launch_fill_u32(c, (uint32_t*)counters, 0u, sizeof(FrameCounters) / 4))) return rc;
+        if ((rc = launch_fill_u32(c, (uint32_t*)image, 0u, px * 3))) return rc;
+        if (save_state) {
+            if ((rc = launch_fill_u32(c, (uint32_t*)tfinal, 0x3F800000u, px))) return rc;
+            if ((rc = launch_fill_u32(c, last, 0u, px))) return rc;
+        }
+        if (!image_override) {
+            c->last_frame_w = p.W;
+            c->last_frame_h = p.H;
+        }
+        c->have_state = save_state;
+        c->state_count = 0;
+        c->state_w = p.W;
+        c->state_h = p.H;
+        c->state_tile = p.tile;
+        c->state_key_bytes = p.key_bytes;
+        c->state_capacity = p.capacity;
+        c->state_sorted_index = 0;
+        return GSR_OK;
+    }
     CamK ck;
     std::memcpy(&ck, cam, sizeof(ck));
     ck.count = count;
-    int rc;
     uint32_t* scan_status = scan_status_buffer(c, count);
     if (!scan_status) return GSR_ERR_ALLOC;
     // one launch clears the counters, the scan's look-back words and the tile ranges
@@ -154,7 +176,7 @@ int gsr_forward(gsr_context* ctx, const float* params_dev, uint32_t count, const
                 const gsr_options* opts, float* out_rgb_dev) {
     Context* c = ctx_of(ctx);
     if (!c) return fail(nullptr, GSR_ERR_ARGUMENT, "null context");
-    if (!params_dev || count == 0) return fail(c, GSR_ERR_ARGUMENT, "gsr_forward: no parameters");
+    if (!params_dev && count > 0) return fail(c, GSR_ERR_ARGUMENT, "gsr_forward: no parameters");
     hipSetDevice(c->device);
     FramePlan p;
     int rc = plan_frame(c, count, cam, opts, &p);
