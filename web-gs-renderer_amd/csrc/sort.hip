@@ -248,12 +248,22 @@ __global__ __launch_bounds__(THREADS) void k_onesweep(
             const uint32_t p = wbase + i * 64 + lane;
             const bool ok = p < n;
             const uint32_t d = (uint32_t)(key[i] >> shift) & mask;
-            uint64_t m = ballot(ok);
-            for (uint32_t b = 0; b < bits; ++b) {
-                const bool bit = (d >> b) & 1u;
-                const uint64_t bb = ballot(bit);
-                m &= bit ? bb : ~bb;
+            // peers: the lanes whose digit equals this lane's, one ballot per
+            // digit bit, unrolled over the digit width (a narrower last pass
+            // has zero high bits everywhere, which keep every lane): keep the
+            // lanes that agree, bb where this lane's bit is 1 and ~bb where it
+            // is 0, i.e. bb ^ (bit - 1)
+            const uint64_t okm = ballot(ok);
+            uint32_t m_lo = (uint32_t)okm, m_hi = (uint32_t)(okm >> 32);
+#pragma unroll
+            for (int b = 0; b < RB; ++b) {
+                const uint32_t bit = (d >> b) & 1u;
+                const uint64_t bb = ballot(bit != 0u);
+                const uint32_t flip = bit - 1u;
+                m_lo &= (uint32_t)bb ^ flip;
+                m_hi &= (uint32_t)(bb >> 32) ^ flip;
             }
+            const uint64_t m = (uint64_t)m_hi << 32 | m_lo;
             const uint32_t below = count_below(m);
             uint32_t prior = 0;
             if (ok) prior = s_wave_cnt[wave][d];
