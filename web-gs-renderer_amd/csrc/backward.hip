@@ -304,14 +304,28 @@ struct SplatShape {
     float R[3][3], M[3][3], S[3][3];  // rotation, R diag(s), 3-D covariance
 };
 
-__device__ __forceinline__ void splat_shape(const float* __restrict__ P, size_t N, uint32_t g, SplatShape& s) {
-    s.px = P[CH_POS * N + g];
-    s.py = P[(CH_POS + 1) * N + g];
-    s.pz = P[(CH_POS + 2) * N + g];
+// One splat's parameter channels: read from the planar scene, or from
+// registers the caller filled.
+struct ParGlobal {
+    const float* __restrict__ P;
+    size_t N;
+    uint32_t g;
+    __device__ float operator()(int ch) const { return P[(size_t)ch * N + g]; }
+};
+struct ParRegs {
+    const float* v;  // the 59 channels
+    __device__ float operator()(int ch) const { return v[ch]; }
+};
+
+template <typename Par>
+__device__ __forceinline__ void splat_shape(const Par& par, SplatShape& s) {
+    s.px = par(CH_POS);
+    s.py = par(CH_POS + 1);
+    s.pz = par(CH_POS + 2);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) s.sv[c] = gsr_exp(P[(CH_SCALE + c) * N + g]);
-    const float rw = P[(CH_ROT + 0) * N + g], rx = P[(CH_ROT + 1) * N + g];
-    const float ry = P[(CH_ROT + 2) * N + g], rz = P[(CH_ROT + 3) * N + g];
+    for (int c = 0; c < 3; ++c) s.sv[c] = gsr_exp(par(CH_SCALE + c));
+    const float rw = par(CH_ROT + 0), rx = par(CH_ROT + 1);
+    const float ry = par(CH_ROT + 2), rz = par(CH_ROT + 3);
     s.ql = sqrtf(((rx * rx + ry * ry) + rz * rz) + rw * rw);
     const float qw = rw / s.ql, qx = rx / s.ql, qy = ry / s.ql, qz = rz / s.ql;
     s.qw = qw;
@@ -340,8 +354,8 @@ __device__ __forceinline__ void splat_shape(const float* __restrict__ P, size_t 
 //   gch   dL/d(SH colour) per channel (zero where the forward clamped it);
 //   basis the 15 higher SH basis values of the view direction;
 // and the view direction's part of dL/dposition is added to gp.
-template <typename Shape>
-__device__ __forceinline__ bool view_adjoint(const float* __restrict__ P, size_t N, uint32_t g, const Shape& s,
+template <typename Par, typename Shape>
+__device__ __forceinline__ bool view_adjoint(const Par& par, const Shape& s,
                                              const CamK& cam, const float (&Gv)[9], float (&gp)[3],
                                              float (&gsym)[3][3], float (&gch)[3], float (&basis)[15]) {
     const float px = s.px, py = s.py, pz = s.pz;
@@ -452,11 +466,11 @@ __device__ __forceinline__ bool view_adjoint(const float* __restrict__ P, size_t
     const float gcol[3] = {Gv[6], Gv[7], Gv[8]};
 #pragma unroll
     for (int ch = 0; ch < 3; ++ch) {
-        float acc = SH_C0 * P[(CH_DC + ch) * N + g];
+        float acc = SH_C0 * par(CH_DC + ch);
         float f[15];
 #pragma unroll
         for (int k = 0; k < 15; ++k) {
-            f[k] = P[(CH_REST + 15 * ch + k) * N + g];
+            f[k] = par(CH_REST + 15 * ch + k);
             acc += bs[k] * f[k];
         }
         const float gv = (acc + 0.5f < 0.0f) ? 0.0f : gcol[ch];
@@ -511,8 +525,9 @@ __device__ __forceinline__ void shape_adjoint(const SplatShape& s, const float (
     gs[6] = (gqz - qz * dotq) * iql;
 }
 
-__device__ __forceinline__ float opacity_of(const float* __restrict__ P, size_t N, uint32_t g) {
-    return 1.0f / (1.0f + gsr_exp(-P[CH_OPACITY * N + g]));
+template <typename Par>
+__device__ __forceinline__ float opacity_of(const Par& par) {
+    return 1.0f / (1.0f + gsr_exp(-par(CH_OPACITY)));
 }
 
 // grad2d arrives with the raster's per-splat sums and leaves in the 2-D
@@ -524,12 +539,19 @@ __global__ __launch_bounds__(256) void k_preprocess_backward(const float* __rest
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
     const size_t N = n;
+    // every load up front (sums, conic, the 59 parameters): one memory round
+    // trip per thread instead of one per dependent stage
     const float4 G0 = grad2d[(size_t)g * 3], G1 = grad2d[(size_t)g * 3 + 1], G2 = grad2d[(size_t)g * 3 + 2];
+    const float4 cn = rec[g].co;
+    float pv[kParams];
+#pragma unroll
+    for (int ch = 0; ch < (int)kParams; ++ch) pv[ch] = P[(size_t)ch * N + g];
+    const ParRegs par{pv};
     const bool any = (G0.x != 0.0f) | (G0.y != 0.0f) | (G0.z != 0.0f) | (G0.w != 0.0f) | (G1.x != 0.0f) |
                      (G1.y != 0.0f) | (G1.z != 0.0f) | (G1.w != 0.0f) | (G2.x != 0.0f);
     float Gv[9];
     if (any) {
-        moments_to_2d(G0, G1, G2, rec[g].co, Gv);
+        moments_to_2d(G0, G1, G2, cn, Gv);
         grad2d[(size_t)g * 3] = make_float4(Gv[0], Gv[1], Gv[2], Gv[3]);
         grad2d[(size_t)g * 3 + 1] = make_float4(Gv[4], G1.y, G1.z, G1.w);
     }
@@ -537,8 +559,8 @@ __global__ __launch_bounds__(256) void k_preprocess_backward(const float* __rest
     float gp[3], gsym[3][3], gch[3], basis[15];
     bool ok = any;
     if (ok) {
-        splat_shape(P, N, g, s);
-        ok = view_adjoint(P, N, g, s, cam, Gv, gp, gsym, gch, basis);
+        splat_shape(par, s);
+        ok = view_adjoint(par, s, cam, Gv, gp, gsym, gch, basis);
     }
     if (!ok) {
         for (int ch = 0; ch < (int)kParams; ++ch) grad[(size_t)ch * N + g] = 0.0f;
@@ -548,7 +570,7 @@ __global__ __launch_bounds__(256) void k_preprocess_backward(const float* __rest
     shape_adjoint(s, gsym, gs);
 #pragma unroll
     for (int k = 0; k < 7; ++k) grad[(size_t)(CH_SCALE + k) * N + g] = gs[k];  // scale, then rotation
-    const float op = opacity_of(P, N, g);
+    const float op = opacity_of(par);
     grad[(size_t)CH_OPACITY * N + g] = Gv[5] * op * (1.0f - op);
 #pragma unroll
     for (int ch = 0; ch < 3; ++ch) {
@@ -612,7 +634,7 @@ __global__ __launch_bounds__(256) void k_preprocess_backward_batch(const float* 
         const float* Pv = opaque(P);  // SH coefficients re-read per view
         if (!shaped) {
             SplatShape full;
-            splat_shape(Pv, N, g, full);
+            splat_shape(ParGlobal{Pv, N, g}, full);
             cv.px = full.px;
             cv.py = full.py;
             cv.pz = full.pz;
@@ -623,7 +645,7 @@ __global__ __launch_bounds__(256) void k_preprocess_backward_batch(const float* 
             shaped = true;
         }
         float gp[3], gsym[3][3], gch[3], basis[15];
-        if (!view_adjoint(Pv, N, g, cv, cams[v], Gv, gp, gsym, gch, basis)) continue;
+        if (!view_adjoint(ParGlobal{Pv, N, g}, cv, cams[v], Gv, gp, gsym, gch, basis)) continue;
         touched = true;
 #pragma unroll
         for (int c = 0; c < 3; ++c) sp[c] += gp[c];
@@ -647,13 +669,13 @@ __global__ __launch_bounds__(256) void k_preprocess_backward_batch(const float* 
     }
     const float* Pe = opaque(P);  // the shape again, not kept live through the loop
     SplatShape s;
-    splat_shape(Pe, N, g, s);
+    splat_shape(ParGlobal{Pe, N, g}, s);
     const float sym[3][3] = {{ss[0], ss[1], ss[2]}, {ss[1], ss[3], ss[4]}, {ss[2], ss[4], ss[5]}};
     float gs[7];
     shape_adjoint(s, sym, gs);
 #pragma unroll
     for (int k = 0; k < 7; ++k) grad[(size_t)(CH_SCALE + k) * N + g] = gs[k];  // scale, then rotation
-    const float op = opacity_of(Pe, N, g);
+    const float op = opacity_of(ParGlobal{Pe, N, g});
     grad[(size_t)CH_OPACITY * N + g] = sop * op * (1.0f - op);
 #pragma unroll
     for (int ch = 0; ch < 3; ++ch) {
