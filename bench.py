@@ -26,7 +26,7 @@ FP32_PEAK_TFLOPS = 157.3
 # HBM bytes per launch of the forward's kernels at this workload, from the
 # rocprofv3 FETCH_SIZE / WRITE_SIZE runs summarised in
 # profiles/forward_1m_1080p_pmc_traffic.md (calibrated against a plain copy).
-MEASURED_TRAFFIC = {"raster": 628783976, "preprocess": 304136736, "sort": 4 * 55914324 + 13873280 + 880032}
+MEASURED_TRAFFIC = {"raster": 620154644, "preprocess": 304120839, "sort": 240074584}
 
 
 def log(msg):
