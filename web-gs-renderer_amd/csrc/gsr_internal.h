@@ -208,10 +208,17 @@ void mark_end(Context* c, bool frame_end = true);
 void finish_frame_timings(Context* c);
 
 // ---- kernel launchers (each enqueues on c->stream) -------------------------
+// A device region of u32 words (zeroed by launch_zero_regions and friends).
+struct FillRegion {
+    uint32_t* p;
+    size_t count;
+};
 // Preprocess writes either the three separate arrays (conic, xy, rgb; the
 // stage entry point) or, when rec is non-null, the packed records.
+// clear (nullable): regions the preprocess grid zeroes on the side.
 int launch_preprocess(Context* c, const float* params, uint32_t n, const CamK& cam, uint32_t tile,
-                      uint32_t* tiles, uint4* meta, float4* conic, float2* xy, float4* rgb, SplatRec* rec);
+                      uint32_t* tiles, uint4* meta, float4* conic, float2* xy, float4* rgb, SplatRec* rec,
+                      const FillRegion* clear = nullptr, int nclear = 0);
 int launch_preprocess_batch(Context* c, const float* params, uint32_t n, const CamK* cams, uint32_t views,
                             uint32_t tile, uint32_t* tiles, uint4* meta, SplatRec* rec);
 // Packs separate (conic, xy, rgb) arrays into records.
@@ -230,11 +237,16 @@ int launch_emit(Context* c, const uint32_t* offsets, const uint4* meta, uint32_t
                 uint64_t capacity, uint32_t key_bytes, void* keys, uint32_t* vals);
 // Sort of n (device counter n_dev if non-null, else n_host) pairs with
 // onesweep blocks_per_cu in {1, 2, 4} and digit_bits in 8..11; returns the
-// result index (which ping-pong buffer holds the output).
+// result index (which ping-pong buffer holds the output).  hist_cleared: an
+// earlier kernel of the stream zeroed sort_hist_region (for the same bit
+// range and digit width), so the sort skips its own clearing launch.
 int launch_sort(Context* c, uint32_t key_bytes, void* k0, uint32_t* v0, void* k1, uint32_t* v1,
                 const uint32_t* n_dev, uint64_t n_host, uint64_t capacity, uint32_t begin_bit,
                 uint32_t end_bit, uint32_t blocks_per_cu, FrameCounters* counters, int* result_index,
-                uint32_t digit_bits = 8);
+                uint32_t digit_bits = 8, bool hist_cleared = false);
+// The sort's histogram and ticket words for this bit range and digit width
+// (count 0 when the buffer cannot be allocated).
+FillRegion sort_hist_region(Context* c, uint32_t begin_bit, uint32_t end_bit, uint32_t digit_bits);
 // The result index launch_sort reports for this bit range and digit width.
 uint32_t sort_result_index(uint32_t begin_bit, uint32_t end_bit, uint32_t digit_bits);
 // Digit width used when the options leave it open: 8 bits for 32-bit keys;
@@ -255,11 +267,36 @@ int launch_fill_u32(Context* c, uint32_t* p, uint32_t value, size_t count);
 int launch_tile_order(Context* c, const uint2* ranges, uint32_t tiles, uint32_t* order,
                       const uint4* views = nullptr, uint32_t num_views = 0);
 // Zeroes up to 4 regions (pointer, u32 count) with one launch.
-struct FillRegion {
-    uint32_t* p;
-    size_t count;
-};
 int launch_zero_regions(Context* c, const FillRegion* regions, int count);
+
+// Up to 4 regions to zero, as a kernel argument: a kernel that runs anyway
+// clears them on the side (zero_regions_part) instead of a launch of its own.
+struct ZeroRegions {
+    uint32_t* p[4];
+    size_t end[4];  // running end of region r in the concatenated index space
+    int count;
+};
+inline ZeroRegions make_zero_regions(const FillRegion* regions, int count) {
+    ZeroRegions z{};
+    size_t total = 0;
+    z.count = count;
+    for (int r = 0; r < count && r < 4; ++r) {
+        z.p[r] = regions[r].p;
+        total += regions[r].count;
+        z.end[r] = total;
+    }
+    return z;
+}
+// Thread t of a grid of `threads` zeroes words t, t + threads, ...
+__device__ __forceinline__ void zero_regions_part(const ZeroRegions& z, size_t t, size_t threads) {
+    if (z.count <= 0) return;
+    const size_t total = z.end[z.count - 1];
+    for (size_t i = t; i < total; i += threads) {
+        int r = 0;
+        while (i >= z.end[r]) ++r;
+        z.p[r][i - (r ? z.end[r - 1] : 0)] = 0u;
+    }
+}
 // Batch raster: views[v] = {W, H, tiles_x, tile_base}, pix[v] = float offset of view v's image.
 // tfinal / last (both or neither): per-pixel state, view v's at pix[v] / 3.
 int launch_raster_queue(Context* c, uint32_t tile, const uint4* views, const uint64_t* pix, uint32_t num_views,

@@ -108,13 +108,18 @@ int enqueue_forward(Context* c, const float* params, uint32_t count, const gsr_c
     ck.count = count;
     uint32_t* scan_status = scan_status_buffer(c, count);
     if (!scan_status) return GSR_ERR_ALLOC;
-    // one launch clears the counters, the scan's look-back words and the tile ranges
-    const FillRegion clear[3] = {{(uint32_t*)counters, sizeof(FrameCounters) / 4},
+    const FillRegion sort_hist = sort_hist_region(c, 0, p.end_bit, p.sort_digits);
+    if (!sort_hist.count) return GSR_ERR_ALLOC;
+    // the preprocess grid also clears the counters, the scan's look-back
+    // words, the tile ranges and the sort's histograms (all read by later
+    // kernels only)
+    const FillRegion clear[4] = {{(uint32_t*)counters, sizeof(FrameCounters) / 4},
                                  {scan_status, scan_status_words(count)},
-                                 {ranges, (size_t)p.num_tiles * 2}};
-    if ((rc = launch_zero_regions(c, clear, 3))) return rc;
+                                 {ranges, (size_t)p.num_tiles * 2},
+                                 sort_hist};
     mark(c, "preprocess");
-    if ((rc = launch_preprocess(c, params, count, ck, p.tile, tiles, meta, nullptr, nullptr, nullptr, rec))) return rc;
+    if ((rc = launch_preprocess(c, params, count, ck, p.tile, tiles, meta, nullptr, nullptr, nullptr, rec, clear, 4)))
+        return rc;
     mark(c, "offset scan");
     if ((rc = launch_scan(c, tiles, count, offsets, counters, p.capacity, true))) return rc;
     mark(c, "emit");
@@ -122,7 +127,7 @@ int enqueue_forward(Context* c, const float* params, uint32_t count, const gsr_c
     mark(c, "sort");
     int idx = 0;
     if ((rc = launch_sort(c, p.key_bytes, k0, v0, k1, v1, &counters->n, 0, p.capacity, 0, p.end_bit,
-                          p.sort_blocks, counters, &idx, p.sort_digits)))
+                          p.sort_blocks, counters, &idx, p.sort_digits, true)))
         return rc;
     mark(c, "tile ranges");
     void* ks = idx ? k1 : k0;

@@ -193,8 +193,10 @@ __global__ __launch_bounds__(256) void k_preprocess(const float* __restrict__ P,
                                                     uint4* __restrict__ meta_out,
                                                     float4* __restrict__ conic_out,
                                                     float2* __restrict__ xy_out,
-                                                    float4* __restrict__ rgb_out, SplatRec* __restrict__ rec_out) {
+                                                    float4* __restrict__ rgb_out, SplatRec* __restrict__ rec_out,
+                                                    ZeroRegions clear) {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    zero_regions_part(clear, g, (size_t)gridDim.x * blockDim.x);
     if (g >= n) return;
     uint32_t ntiles;
     uint4 meta;
@@ -292,14 +294,17 @@ int launch_preprocess_batch(Context* c, const float* params, uint32_t n, const C
 }
 
 int launch_preprocess(Context* c, const float* params, uint32_t n, const CamK& cam, uint32_t tile,
-                      uint32_t* tiles, uint4* meta, float4* conic, float2* xy, float4* rgb, SplatRec* rec) {
-    if (n == 0) return GSR_OK;
+                      uint32_t* tiles, uint4* meta, float4* conic, float2* xy, float4* rgb, SplatRec* rec,
+                      const FillRegion* clear, int nclear) {
+    if (nclear < 0 || nclear > 4) return fail(c, GSR_ERR_ARGUMENT, "preprocess: %d regions to clear", nclear);
+    if (n == 0) return clear && nclear ? launch_zero_regions(c, clear, nclear) : GSR_OK;
+    const ZeroRegions z = make_zero_regions(clear, clear ? nclear : 0);
     if (rec)
         hipLaunchKernelGGL(k_preprocess<true>, dim3(ceil_div(n, 256)), dim3(256), 0, c->stream, params, n, cam, tile,
-                           tiles, meta, nullptr, nullptr, nullptr, rec);
+                           tiles, meta, nullptr, nullptr, nullptr, rec, z);
     else
         hipLaunchKernelGGL(k_preprocess<false>, dim3(ceil_div(n, 256)), dim3(256), 0, c->stream, params, n, cam, tile,
-                           tiles, meta, conic, xy, rgb, nullptr);
+                           tiles, meta, conic, xy, rgb, nullptr, z);
     GSR_HIP(c, hipGetLastError());
     return GSR_OK;
 }

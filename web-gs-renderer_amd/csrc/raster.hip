@@ -69,19 +69,8 @@ __global__ __launch_bounds__(256) void k_fill_u32(uint32_t* __restrict__ p, uint
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) p[i] = v;
 }
 
-struct ZeroRegions {
-    uint32_t* p[4];
-    size_t end[4];  // running end of region r in the concatenated index space
-};
-
-__global__ __launch_bounds__(256) void k_zero_regions(ZeroRegions z, int count) {
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    const size_t total = z.end[count - 1];
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
-        int r = 0;
-        while (i >= z.end[r]) ++r;
-        z.p[r][i - (r ? z.end[r - 1] : 0)] = 0u;
-    }
+__global__ __launch_bounds__(256) void k_zero_regions(ZeroRegions z) {
+    zero_regions_part(z, (size_t)blockIdx.x * blockDim.x + threadIdx.x, (size_t)gridDim.x * blockDim.x);
 }
 
 // One gathered splat as staged in LDS: 48 bytes, read back with three
@@ -314,16 +303,11 @@ int launch_tile_order(Context* c, const uint2* ranges, uint32_t tiles, uint32_t*
 
 int launch_zero_regions(Context* c, const FillRegion* regions, int count) {
     if (count < 1 || count > 4) return fail(c, GSR_ERR_ARGUMENT, "zero regions: %d regions", count);
-    ZeroRegions z{};
-    size_t total = 0;
-    for (int r = 0; r < count; ++r) {
-        z.p[r] = regions[r].p;
-        total += regions[r].count;
-        z.end[r] = total;
-    }
+    const ZeroRegions z = make_zero_regions(regions, count);
+    const size_t total = z.end[count - 1];
     if (total == 0) return GSR_OK;
     const uint32_t blocks = (uint32_t)std::min<size_t>((total + 255) / 256, 2048);
-    hipLaunchKernelGGL(k_zero_regions, dim3(blocks), dim3(256), 0, c->stream, z, count);
+    hipLaunchKernelGGL(k_zero_regions, dim3(blocks), dim3(256), 0, c->stream, z);
     GSR_HIP(c, hipGetLastError());
     return GSR_OK;
 }

@@ -4,12 +4,13 @@
 // driven by src/renderer/renderer.js:869-917).  Same result: a stable sort on
 // the key bits [begin_bit, end_bit), hence the (key, value)-lexicographic
 // order of the emitted instances.  Structure per sort:
-//   k_sort_init      zero digit histograms, tile tickets and look-back words
-//                    (only for the tiles the device-side count needs);
-//   k_sort_hist      one read of the keys builds every pass's digit
-//                    histogram (LDS bins; a run of equal digits in
-//                    consecutive lanes adds once; one global atomic per bin
-//                    and block, one block per CU);
+//   k_sort_init      zero digit histograms and tile tickets (skipped when
+//                    the frame's preprocess grid already cleared them);
+//   k_sort_hist      zero the first pass's look-back words (only for the
+//                    tiles the device-side count needs); one read of the
+//                    keys builds every pass's digit histogram (LDS bins; a
+//                    run of equal digits in consecutive lanes adds once; one
+//                    global atomic per bin and block, one block per CU);
 //   k_onesweep x P   persistent blocks take tiles by ticket, rank keys per
 //                    wave64 with ballot digit matching (stable), chain
 //                    per-digit tile prefixes with decoupled look-back, stage
@@ -108,19 +109,12 @@ __device__ __forceinline__ void digits_exclusive(const uint32_t (&v)[DPT], uint3
     for (int k = 0; k < DPT; ++k) out[k] += base;
 }
 
-// Zeroes the histograms and tickets of every pass, and the look-back words
-// of the first pass's tiles (the later passes clear their successors' words
-// themselves).
-__global__ __launch_bounds__(256) void k_sort_init(uint32_t* __restrict__ hist_tickets, size_t hist_words,
-                                                   uint32_t* __restrict__ status, const uint32_t* n_dev,
-                                                   uint64_t n_host, uint32_t tile_keys, uint32_t radix) {
-    const uint32_t n = read_n(n_dev, n_host);
-    const uint32_t tiles = ceil_div(n, tile_keys);
+// Zeroes the histograms and tickets of every pass (unless the caller had a
+// kernel of its own do it; the histogram kernel clears the first pass's
+// look-back words, and each pass its successor's).
+__global__ __launch_bounds__(256) void k_sort_init(uint32_t* __restrict__ hist_tickets, size_t hist_words) {
     const size_t stride = (size_t)gridDim.x * blockDim.x;
-    const size_t t0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    for (size_t i = t0; i < hist_words; i += stride) hist_tickets[i] = 0;
-    const size_t words = (size_t)tiles * radix;
-    for (size_t i = t0; i < words; i += stride) status[i] = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < hist_words; i += stride) hist_tickets[i] = 0;
 }
 
 // 1024-thread blocks (one per CU) take 1024 x kHistItems-key tiles in a
@@ -131,12 +125,17 @@ __global__ __launch_bounds__(256) void k_sort_init(uint32_t* __restrict__ hist_t
 template <typename K, int RB>
 __global__ __launch_bounds__(1024) void k_sort_hist(const K* __restrict__ keys, const uint32_t* n_dev,
                                                    uint64_t n_host, uint32_t begin_bit, uint32_t end_bit,
-                                                   uint32_t passes, uint32_t* __restrict__ hist) {
+                                                   uint32_t passes, uint32_t* __restrict__ hist,
+                                                   uint32_t* __restrict__ status0, uint32_t tile_keys) {
     constexpr int R = 1 << RB;
     constexpr int COPIES = RB <= 8 ? 4 : 1;
     constexpr int MAXP = (int)(sizeof(K) * 8 + RB - 1) / RB;
     __shared__ uint32_t s_hist[COPIES][MAXP * R];
     const uint32_t n = read_n(n_dev, n_host);
+    {   // the first onesweep pass's look-back words, for the tiles of n
+        const size_t words = (size_t)ceil_div(n, tile_keys) * R, stride = (size_t)gridDim.x * blockDim.x;
+        for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += stride) status0[i] = 0;
+    }
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, copy = wave % COPIES;
     for (uint32_t w = 0; w < COPIES; ++w)
         for (uint32_t i = threadIdx.x; i < passes * R; i += blockDim.x) s_hist[w][i] = 0;
@@ -373,7 +372,7 @@ __global__ __launch_bounds__(THREADS) void k_onesweep(
 template <typename K, int THREADS, int RB>
 int sort_impl(Context* c, K* k0, uint32_t* v0, K* k1, uint32_t* v1, const uint32_t* n_dev, uint64_t n_host,
               uint64_t capacity, uint32_t begin_bit, uint32_t end_bit, FrameCounters* counters,
-              int* result_index) {
+              int* result_index, bool hist_cleared) {
     using G = SortGeom<K, THREADS, RB>;
     constexpr uint32_t R = G::kRadixN;
     const uint32_t passes = sort_passes(begin_bit, end_bit, RB);
@@ -387,16 +386,16 @@ int sort_impl(Context* c, K* k0, uint32_t* v0, K* k1, uint32_t* v1, const uint32
     uint32_t* status = (uint32_t*)scratch(c, "sort_status", (size_t)2 * max_tiles * R * 4);
     if (!hist || !status) return GSR_ERR_ALLOC;
     uint32_t* tickets = hist + (size_t)passes * R;
-    const uint32_t fill_blocks = std::min<uint32_t>(std::max<uint32_t>(1, max_tiles), 1024);
-    hipLaunchKernelGGL(k_sort_init, dim3(fill_blocks), dim3(256), 0, c->stream, hist, hist_words, status, n_dev,
-                       n_host, (uint32_t)G::kTile, R);
+    if (!hist_cleared)
+        hipLaunchKernelGGL(k_sort_init, dim3((uint32_t)std::min<size_t>((hist_words + 255) / 256, 64)), dim3(256), 0,
+                           c->stream, hist, hist_words);
     // One block per CU: every block adds its bins to the global histogram with
     // one atomic each, and those atomics serialize per bin across blocks, so
     // fewer, larger blocks.
     const uint32_t hist_blocks = std::min<uint32_t>(
         std::max<uint32_t>(1, ceil_div(capacity, (uint64_t)1024 * kHistItems)), (uint32_t)c->cu_count);
     hipLaunchKernelGGL((k_sort_hist<K, RB>), dim3(hist_blocks), dim3(1024), 0, c->stream, k0, n_dev, n_host,
-                       begin_bit, end_bit, passes, hist);
+                       begin_bit, end_bit, passes, hist, status, (uint32_t)G::kTile);
     const uint32_t sweep_blocks = std::min<uint32_t>(max_tiles, (uint32_t)c->cu_count * (1024 / THREADS));
     K* kb[2] = {k0, k1};
     uint32_t* vb[2] = {v0, v1};
@@ -416,17 +415,17 @@ int sort_impl(Context* c, K* k0, uint32_t* v0, K* k1, uint32_t* v1, const uint32
 template <typename K, int RB>
 int sort_geometry(Context* c, K* k0, uint32_t* v0, K* k1, uint32_t* v1, const uint32_t* n_dev, uint64_t n_host,
                   uint64_t capacity, uint32_t begin_bit, uint32_t end_bit, uint32_t blocks_per_cu,
-                  FrameCounters* counters, int* result_index) {
+                  FrameCounters* counters, int* result_index, bool hist_cleared) {
     switch (blocks_per_cu) {
         case 1:
             return sort_impl<K, 1024, RB>(c, k0, v0, k1, v1, n_dev, n_host, capacity, begin_bit, end_bit, counters,
-                                          result_index);
+                                          result_index, hist_cleared);
         case 2:
             return sort_impl<K, 512, RB>(c, k0, v0, k1, v1, n_dev, n_host, capacity, begin_bit, end_bit, counters,
-                                         result_index);
+                                          result_index, hist_cleared);
         case 4:
             return sort_impl<K, 256, RB>(c, k0, v0, k1, v1, n_dev, n_host, capacity, begin_bit, end_bit, counters,
-                                         result_index);
+                                          result_index, hist_cleared);
         default:
             return fail(c, GSR_ERR_ARGUMENT, "sort blocks per CU %u not in {1, 2, 4}", blocks_per_cu);
     }
@@ -435,20 +434,20 @@ int sort_geometry(Context* c, K* k0, uint32_t* v0, K* k1, uint32_t* v1, const ui
 template <typename K>
 int sort_digits(Context* c, K* k0, uint32_t* v0, K* k1, uint32_t* v1, const uint32_t* n_dev, uint64_t n_host,
                 uint64_t capacity, uint32_t begin_bit, uint32_t end_bit, uint32_t blocks_per_cu, uint32_t digit_bits,
-                FrameCounters* counters, int* result_index) {
+                FrameCounters* counters, int* result_index, bool hist_cleared) {
     switch (digit_bits) {
         case 8:
             return sort_geometry<K, 8>(c, k0, v0, k1, v1, n_dev, n_host, capacity, begin_bit, end_bit, blocks_per_cu,
-                                       counters, result_index);
+                                       counters, result_index, hist_cleared);
         case 9:
             return sort_geometry<K, 9>(c, k0, v0, k1, v1, n_dev, n_host, capacity, begin_bit, end_bit, blocks_per_cu,
-                                       counters, result_index);
+                                       counters, result_index, hist_cleared);
         case 10:
             return sort_geometry<K, 10>(c, k0, v0, k1, v1, n_dev, n_host, capacity, begin_bit, end_bit,
-                                        blocks_per_cu, counters, result_index);
+                                        blocks_per_cu, counters, result_index, hist_cleared);
         case 11:
             return sort_geometry<K, 11>(c, k0, v0, k1, v1, n_dev, n_host, capacity, begin_bit, end_bit,
-                                        blocks_per_cu, counters, result_index);
+                                        blocks_per_cu, counters, result_index, hist_cleared);
         default:
             return fail(c, GSR_ERR_ARGUMENT, "sort digit width %u not in 8..11", digit_bits);
     }
@@ -470,19 +469,25 @@ uint32_t sort_result_index(uint32_t begin_bit, uint32_t end_bit, uint32_t digit_
     return sort_passes(begin_bit, end_bit, digit_bits) & 1u;
 }
 
+FillRegion sort_hist_region(Context* c, uint32_t begin_bit, uint32_t end_bit, uint32_t digit_bits) {
+    const size_t words = (size_t)sort_passes(begin_bit, end_bit, digit_bits) * (1u << digit_bits) + kMaxPasses;
+    uint32_t* p = (uint32_t*)scratch(c, "sort_hist", words * 4);
+    return FillRegion{p, p ? words : 0};
+}
+
 int launch_sort(Context* c, uint32_t key_bytes, void* k0, uint32_t* v0, void* k1, uint32_t* v1,
                 const uint32_t* n_dev, uint64_t n_host, uint64_t capacity, uint32_t begin_bit,
                 uint32_t end_bit, uint32_t blocks_per_cu, FrameCounters* counters, int* result_index,
-                uint32_t digit_bits) {
+                uint32_t digit_bits, bool hist_cleared) {
     if (capacity > kMaxSortKeys) return fail(c, GSR_ERR_ARGUMENT, "sort capacity %llu exceeds %llu",
                                              (unsigned long long)capacity, (unsigned long long)kMaxSortKeys);
     if (end_bit > key_bytes * 8 || begin_bit > end_bit)
         return fail(c, GSR_ERR_ARGUMENT, "bad bit range [%u, %u) for %u-byte keys", begin_bit, end_bit, key_bytes);
     if (key_bytes == 8)
         return sort_digits<uint64_t>(c, (uint64_t*)k0, v0, (uint64_t*)k1, v1, n_dev, n_host, capacity, begin_bit,
-                                     end_bit, blocks_per_cu, digit_bits, counters, result_index);
+                                     end_bit, blocks_per_cu, digit_bits, counters, result_index, hist_cleared);
     return sort_digits<uint32_t>(c, (uint32_t*)k0, v0, (uint32_t*)k1, v1, n_dev, n_host, capacity, begin_bit,
-                                 end_bit, blocks_per_cu, digit_bits, counters, result_index);
+                                 end_bit, blocks_per_cu, digit_bits, counters, result_index, hist_cleared);
 }
 
 }  // namespace gsr
