@@ -21,8 +21,11 @@
 // frame key 3 at 10 or 11 bits), and as many items per thread as the CU's
 // LDS holds (SortGeom).  Large tiles keep the look-back
 // chains short: a tile's look-back walks back through the tiles that
-// started with it, 16 words per round trip, so the count of tiles in flight
-// sets its latency.
+// started with it, so the count of tiles in flight sets its latency.  Where
+// the block has more threads than digits, the threads of one digit share its
+// look-back (4 predecessors per thread and round trip, the parts combined in
+// order with lane shuffles); otherwise each digit's thread loads 16 per round
+// trip.
 // The key count may live on the device (the frame pipeline), so every grid is
 // sized from the capacity and blocks exit on the device count: no host
 // readback, capturable in a hipGraph.
@@ -65,6 +68,11 @@ struct SortGeom {
     static constexpr int kTile = THREADS * kItems;
     static_assert(lds(kItems) <= kLdsBytes, "onesweep geometry does not fit LDS");
     static constexpr int kLB = kLookBack / kDPT > 4 ? kLookBack / kDPT : 4;  // predecessors per step
+    // threads sharing one digit's look-back when the block has more threads
+    // than digits, each loading its own kLBH predecessors of every step: few
+    // loads per lane return soonest (2 to 4 measured best, 8 or more slower)
+    static constexpr int kH = THREADS >= kRadixN ? THREADS / kRadixN : 1;
+    static constexpr int kLBH = 4;
     static constexpr int kDigitWaves = (kDigitThreads + 63) / 64;
 };
 
@@ -191,7 +199,8 @@ __global__ __launch_bounds__(THREADS) void k_onesweep(
     uint32_t* error) {
     using G = SortGeom<K, THREADS, RB>;
     constexpr int WAVES = G::kWaves, ITEMS = G::kItems, TILE = G::kTile, R = G::kRadixN, DPT = G::kDPT,
-                  LB = G::kLB;
+                  LB = G::kLB, H = G::kH, LBH = G::kLBH;
+    static_assert(H == 1 || DPT == 1, "shared look-back needs one digit per owner thread");
     using CntT = typename G::CntT;
     __shared__ __align__(16) CntT s_wave_cnt[WAVES][R];
     __shared__ uint32_t s_global[R];
@@ -288,10 +297,66 @@ __global__ __launch_bounds__(THREADS) void k_onesweep(
         }
         uint32_t local[DPT];
         digits_exclusive<G::kDigitWaves, DPT>(cnt, local, s_w);
+        if constexpr (H > 1) {
+            // H adjacent lanes per digit: lane q of digit d loads predecessors
+            // q * LBH + 1 .. (q + 1) * LBH back of each step of H * LBH (one
+            // round trip), sums them nearest first up to the first inclusive
+            // word, and the H partial results combine in order.  s_scatter
+            // holds the digits' tile counts until the offsets replace them.
+            if (digit_thread) {
+                s_local[d0] = local[0];
+                s_scatter[d0] = cnt[0];
+            }
+            __syncthreads();
+            const uint32_t d = tid / H, q = tid % H;
+            uint32_t ex = 0;
+            if (tile > 0) {
+                int64_t j = (int64_t)tile - 1 - (int64_t)q * LBH;  // this lane's nearest predecessor
+                uint32_t spins = 0;
+                for (;;) {
+                    uint32_t w[LBH];
+#pragma unroll
+                    for (int t = 0; t < LBH; ++t)
+                        w[t] = (j - t >= 0) ? ld_acquire_u32(status + (size_t)(j - t) * R + d) : kInc;
+                    uint32_t sum = 0, found = 0;
+#pragma unroll
+                    for (int t = 0; t < LBH; ++t) {
+                        if (found) continue;
+                        uint32_t s = w[t];
+                        while ((s >> 30) == 0) {
+                            if (++spins > kSpinLimit) {
+                                atomicOr(error, 2u);
+                                s = kInc;
+                                break;
+                            }
+                            __builtin_amdgcn_s_sleep(1);
+                            s = ld_acquire_u32(status + (size_t)(j - t) * R + d);
+                        }
+                        sum += s & kStatusValueMask;
+                        found = (s & kInc) ? 1u : 0u;
+                    }
+                    // ordered combine: the nearer part's sum ends at its
+                    // inclusive word, else the farther part's sum adds on
+#pragma unroll
+                    for (int o = 1; o < H; o <<= 1) {
+                        const uint32_t ps = __shfl_xor(sum, o, H), pf = __shfl_xor(found, o, H);
+                        const bool near = (q & (uint32_t)o) == 0;
+                        const uint32_t nf = near ? found : pf;
+                        sum = nf ? (near ? sum : ps) : sum + ps;
+                        found |= pf;
+                    }
+                    ex += sum;
+                    if (found) break;
+                    j -= (int64_t)H * LBH;
+                }
+                if (q == 0) st_release_u32(status + (size_t)tile * R + d, kInc | ((ex + s_scatter[d]) & kStatusValueMask));
+            }
+            if (q == 0) s_scatter[d] = s_global[d] + ex - s_local[d];
+        }
         uint32_t excl[DPT];
 #pragma unroll
         for (int k = 0; k < DPT; ++k) excl[k] = 0;
-        if (digit_thread && tile > 0) {
+        if (H == 1 && digit_thread && tile > 0) {
             // Batched look-back: the words of up to LB predecessors (all of
             // this thread's digits) are loaded together (independent loads,
             // one round trip), then consumed nearest first; only a word still
@@ -330,7 +395,7 @@ __global__ __launch_bounds__(THREADS) void k_onesweep(
             for (int k = 0; k < DPT; ++k)
                 st_release_u32(my_status + k, kInc | ((excl[k] + cnt[k]) & kStatusValueMask));
         }
-        if (digit_thread) {
+        if (H == 1 && digit_thread) {
 #pragma unroll
             for (int k = 0; k < DPT; ++k) {
                 s_local[d0 + k] = local[k];
