@@ -143,7 +143,9 @@ __device__ __forceinline__ void backward_block(uint32_t W, uint32_t H, uint32_t 
     const uint32_t tag = fold_tags();
     const float rxa = (float)x0, rxb = (float)min(x0 + 7, W - 1);
     const float rya = (float)y0, ryb = (float)min(y0 + 7, H - 1);
-    SplatB* sw = s_spl[threadIdx.x >> 6];
+    // the wave's slot row as a wave-uniform (scalar) index, so that a
+    // record's LDS address is scalar arithmetic
+    const uint32_t wv = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     for (int64_t b = (int64_t)start + (int64_t)((stop - start - 1) / 64) * 64; b >= (int64_t)start; b -= 64) {
         const uint32_t idx = (uint32_t)b + lane;
         float4 co = make_float4(0.f, 0.f, 0.f, 0.f), cl = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -163,9 +165,9 @@ __device__ __forceinline__ void backward_block(uint32_t W, uint32_t H, uint32_t 
         }
         // Stage the batch in this wave's LDS slots (as in the forward).
         __builtin_amdgcn_wave_barrier();
-        sw[lane].co = co;
-        sw[lane].cl = make_float4(cl.x, cl.y, cl.z, __uint_as_float(gid));
-        sw[lane].p = p;
+        s_spl[wv][lane].co = co;
+        s_spl[wv][lane].cl = make_float4(cl.x, cl.y, cl.z, __uint_as_float(gid));
+        s_spl[wv][lane].p = p;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -174,7 +176,7 @@ __device__ __forceinline__ void backward_block(uint32_t W, uint32_t H, uint32_t 
             const int j = 63 - (int)__builtin_clzll(lm);
             lm &= ~(1ull << j);
             const uint32_t pos = (uint32_t)b - start + (uint32_t)j + 1;
-            const SplatB s = sw[j];
+            const SplatB s = s_spl[wv][j];
             const float A = s.co.x, B = s.co.y, C = s.co.z, op = s.co.w;  // A..C: power only
             const float cr = s.cl.x, cg = s.cl.y, cb = s.cl.z;
             const float dx = s.p.x - fx, dy = s.p.y - fy;

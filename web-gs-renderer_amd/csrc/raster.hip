@@ -94,6 +94,9 @@ __device__ __forceinline__ void raster_block(uint32_t W, uint32_t H, uint32_t x0
                                             float* __restrict__ tfinal, uint32_t* __restrict__ last) {
     __shared__ SplatL s_spl[4][64];
     const uint32_t lane = threadIdx.x & 63;
+    // the wave's slot row as a wave-uniform (scalar) index, so that a
+    // record's LDS address is scalar arithmetic
+    const uint32_t wv = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const uint32_t x = x0 + lane % 8, y = y0 + lane / 8;
     const bool inside = x < W && y < H;
     bool done = !inside;
@@ -121,11 +124,10 @@ __device__ __forceinline__ void raster_block(uint32_t W, uint32_t H, uint32_t x0
         // Stage the batch in this wave's LDS slots.  Only this wave touches
         // them: a wave-level barrier orders the previous batch's reads before
         // these writes and these writes before the reads below.
-        SplatL* sw = s_spl[threadIdx.x >> 6];
         __builtin_amdgcn_wave_barrier();
-        sw[lane].co = co;
-        sw[lane].cl = cl;
-        sw[lane].p = p;
+        s_spl[wv][lane].co = co;
+        s_spl[wv][lane].cl = cl;
+        s_spl[wv][lane].p = p;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -137,7 +139,7 @@ __device__ __forceinline__ void raster_block(uint32_t W, uint32_t H, uint32_t x0
         // it unchanged bit for bit (colours are finite and >= 0).
         for (uint64_t lm = ballot(live); lm != 0; lm &= lm - 1) {
             const int j = (int)__builtin_ctzll(lm);
-            const SplatL s = sw[j];
+            const SplatL s = s_spl[wv][j];
             const float dx = s.p.x - fx, dy = s.p.y - fy;
             const float q = __builtin_fmaf(s.co.x * dx, dx, (s.co.z * dy) * dy);
             const float power = __builtin_fmaf(-0.5f, q, -((s.co.y * dx) * dy));
