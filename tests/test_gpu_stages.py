@@ -180,6 +180,25 @@ def test_sort_geometries_stable_exact(torch_cuda, ctx, blocks, digits, n, key_by
     assert np.array_equal(k, keys[order])
 
 
+@pytest.mark.parametrize("n,key_bytes,bits,digits", [
+    (1_500_000, 4, 29, 8), (2_200_000, 4, 29, 8), (3_000_000, 4, 29, 8), (3_400_000, 4, 29, 8),
+    (4_500_000, 4, 29, 8), (1_300_000, 8, 33, 9), (1_700_000, 8, 33, 9), (2_200_000, 8, 33, 9)])
+def test_sort_tile_lengths(torch_cuda, ctx, n, key_bytes, bits, digits):
+    """Key counts between one and a few full tiles per resident block, where
+    the onesweep tile length follows the device-side count (each of the
+    shortened lengths of the default 32- and 64-bit geometries on a 256-CU
+    part, and the full one): still the stable sort."""
+    _say(f"sort tile lengths n={n} key_bytes={key_bytes} bits={bits} digits={digits}")
+    rng = np.random.default_rng(n + digits)
+    keys = rng.integers(0, (1 << bits) - 1, size=n, dtype=np.uint64, endpoint=True)
+    keys[: n // 3] = keys[rng.integers(0, 64, n // 3)]   # ties
+    vals = rng.permutation(n).astype(np.uint32)
+    k, v = _gpu_sort(torch_cuda, ctx, keys, vals, key_bytes, 0, bits, 1, digits)
+    order = np.argsort(keys, kind="stable")
+    assert np.array_equal(v, vals[order])
+    assert np.array_equal(k, keys[order])
+
+
 def test_sort_geometry_rejects_unknown_widths(torch_cuda, ctx):
     keys = np.arange(10, dtype=np.uint64)
     for blocks, digits in ((3, 8), (1, 7), (1, 12)):

@@ -18,10 +18,14 @@
 // Geometry: THREADS = 1024 / blocks-per-CU threads (1, 2 or 4 blocks per
 // CU), digits of 8 to 11 bits (256 to 2048 buckets: a 33- or 34-bit
 // view|tile|depth key takes 4 passes at 9 bits instead of 5 at 8, a 29-bit
-// frame key 3 at 10 or 11 bits), and as many items per thread as the CU's
-// LDS holds (SortGeom).  Large tiles keep the look-back
+// frame key 3 at 10 or 11 bits), and up to as many items per thread as the
+// CU's LDS holds (SortGeom).  Large tiles keep the look-back
 // chains short: a tile's look-back walks back through the tiles that
-// started with it, so the count of tiles in flight sets its latency.  Where
+// started with it, so the count of tiles in flight sets its latency.  Below
+// one full tile per resident block the tiles shrink to about one per block
+// (onesweep_items, from the device-side count; each length is its own
+// fully unrolled instantiation): a 3.3M-key frame sort then runs 14 items
+// per thread in one round of 233 tiles, a 1M-key one 4 items in 245.  Where
 // the block has more threads than digits, the threads of one digit share its
 // look-back (4 predecessors per thread and round trip, the parts combined in
 // order with lane shuffles); otherwise each digit's thread loads 16 per round
@@ -43,8 +47,8 @@ constexpr int kHistItems = 16;  // keys per thread per histogram block tile
 constexpr size_t kLdsBytes = 160 * 1024;
 
 // Onesweep geometry for keys K, THREADS threads per block and RB-bit digits.
-// Each thread ranks kItems keys; the tile is THREADS * kItems keys, the
-// largest of 16 / 8 / 4 items whose LDS (keys and values staged, 16-bit
+// Each thread ranks up to kItems keys; the full tile is THREADS * kItems
+// keys, the largest of 16 / 8 / 4 items whose LDS (keys and values staged, 16-bit
 // per-wave digit counts, three digit tables) fits one CU.  Digits are owned
 // kDPT per thread (thread t: digits t * kDPT ...), so 2048 digits need no
 // more than 512 threads.
@@ -83,6 +87,27 @@ __host__ __device__ constexpr uint32_t sort_passes(uint32_t begin_bit, uint32_t 
 
 __device__ __forceinline__ uint32_t read_n(const uint32_t* n_dev, uint64_t n_host) {
     return n_dev ? *n_dev : (uint32_t)n_host;
+}
+
+// Keys per thread of a onesweep tile for n keys and `slots` resident blocks
+// of `threads`: the full max_items once n fills every block's tile, else
+// the shortest of the tile lengths below that still gives at most one tile
+// per block.  While a pass is one round of tiles, its time is one tile's
+// chain (load, rank, look-back, scatter), which shorter tiles shorten; a
+// second round would cost a whole chain more.  Choice k of max_items m:
+// m, 7m/8, 3m/4, m/2, m/4 (at least 1).
+__host__ __device__ constexpr uint32_t onesweep_item_choice(uint32_t m, int k) {
+    const uint32_t c = k == 0 ? m : k == 1 ? m * 7 / 8 : k == 2 ? m * 3 / 4 : k == 3 ? m / 2 : m / 4;
+    return c < 1 ? 1u : c;
+}
+__host__ __device__ constexpr uint32_t onesweep_items(uint32_t n, uint32_t slots, uint32_t threads,
+                                                      uint32_t max_items) {
+    const uint64_t per_item = (uint64_t)slots * threads;
+    const uint64_t need = per_item ? (n + per_item - 1) / per_item : max_items;
+    uint32_t items = max_items;
+    for (int k = 1; k <= 4; ++k)
+        if (onesweep_item_choice(max_items, k) >= need) items = onesweep_item_choice(max_items, k);
+    return items;
 }
 
 // Exclusive prefix of v over the first WAVES waves of the block, in thread
@@ -134,13 +159,15 @@ template <typename K, int RB>
 __global__ __launch_bounds__(1024) void k_sort_hist(const K* __restrict__ keys, const uint32_t* n_dev,
                                                    uint64_t n_host, uint32_t begin_bit, uint32_t end_bit,
                                                    uint32_t passes, uint32_t* __restrict__ hist,
-                                                   uint32_t* __restrict__ status0, uint32_t tile_keys) {
+                                                   uint32_t* __restrict__ status0, uint32_t slots,
+                                                   uint32_t sweep_threads, uint32_t max_items) {
     constexpr int R = 1 << RB;
     constexpr int COPIES = RB <= 8 ? 4 : 1;
     constexpr int MAXP = (int)(sizeof(K) * 8 + RB - 1) / RB;
     __shared__ uint32_t s_hist[COPIES][MAXP * R];
     const uint32_t n = read_n(n_dev, n_host);
     {   // the first onesweep pass's look-back words, for the tiles of n
+        const uint32_t tile_keys = onesweep_items(n, slots, sweep_threads, max_items) * sweep_threads;
         const size_t words = (size_t)ceil_div(n, tile_keys) * R, stride = (size_t)gridDim.x * blockDim.x;
         for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += stride) status0[i] = 0;
     }
@@ -191,68 +218,56 @@ __global__ __launch_bounds__(1024) void k_sort_hist(const K* __restrict__ keys, 
 // One pass.  status: this pass's look-back words (tile * R + digit), zero
 // for every tile of the key count; clear_next: the next pass's words, which
 // this pass zeroes (null on the last pass).
+// LDS of a onesweep block (one allocation whatever the tile length).
 template <typename K, int THREADS, int RB>
-__global__ __launch_bounds__(THREADS) void k_onesweep(
-    const K* __restrict__ kin, const uint32_t* __restrict__ vin, K* __restrict__ kout,
-    uint32_t* __restrict__ vout, const uint32_t* n_dev, uint64_t n_host, uint32_t shift, uint32_t bits,
-    const uint32_t* __restrict__ hist, uint32_t* ticket, uint32_t* status, uint32_t* clear_next,
-    uint32_t* error) {
+struct OnesweepLds {
     using G = SortGeom<K, THREADS, RB>;
-    constexpr int WAVES = G::kWaves, ITEMS = G::kItems, TILE = G::kTile, R = G::kRadixN, DPT = G::kDPT,
-                  LB = G::kLB, H = G::kH, LBH = G::kLBH;
-    static_assert(H == 1 || DPT == 1, "shared look-back needs one digit per owner thread");
-    using CntT = typename G::CntT;
-    __shared__ __align__(16) CntT s_wave_cnt[WAVES][R];
-    __shared__ uint32_t s_global[R];
-    __shared__ uint32_t s_local[R];
-    __shared__ uint32_t s_scatter[R];
-    __shared__ K s_keys[TILE];
-    __shared__ uint32_t s_vals[TILE];
-    __shared__ uint32_t s_w[WAVES];
-    __shared__ uint32_t s_tile;
+    __align__(16) typename G::CntT s_wave_cnt[G::kWaves][G::kRadixN];
+    uint32_t s_global[G::kRadixN];
+    uint32_t s_local[G::kRadixN];
+    uint32_t s_scatter[G::kRadixN];
+    K s_keys[G::kTile];
+    uint32_t s_vals[G::kTile];
+    uint32_t s_w[G::kWaves];
+    uint32_t s_tile;
+};
 
+// The tiles of one pass, IT keys per thread (a compile-time count, so the
+// item loops unroll fully; k_onesweep picks the instantiation).
+template <typename K, int THREADS, int RB, int IT>
+__device__ __forceinline__ void onesweep_tiles(OnesweepLds<K, THREADS, RB>& L, const K* __restrict__ kin,
+                                               const uint32_t* __restrict__ vin, K* __restrict__ kout,
+                                               uint32_t* __restrict__ vout, uint32_t n, uint32_t num_tiles,
+                                               uint32_t shift, uint32_t mask, uint32_t* ticket,
+                                               uint32_t* status, uint32_t* error) {
+    using G = SortGeom<K, THREADS, RB>;
+    constexpr int WAVES = G::kWaves, R = G::kRadixN, DPT = G::kDPT, LB = G::kLB, H = G::kH, LBH = G::kLBH;
+    using CntT = typename G::CntT;
+    constexpr uint32_t tile_keys = (uint32_t)IT * THREADS;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const bool digit_thread = tid < (uint32_t)G::kDigitThreads;
     const uint32_t d0 = tid * DPT;  // first digit of this thread
-    const uint32_t n = read_n(n_dev, n_host);
-    const uint32_t num_tiles = ceil_div(n, (uint32_t)TILE);
-    const uint32_t mask = (1u << bits) - 1u;
-    if (clear_next) {
-        const size_t words = (size_t)num_tiles * R, stride = (size_t)gridDim.x * THREADS;
-        for (size_t i = (size_t)blockIdx.x * THREADS + tid; i < words; i += stride) clear_next[i] = 0;
-    }
-    if (blockIdx.x >= num_tiles) return;  // uniform per block
-    if (tid == 0) s_tile = atomicAdd(ticket, 1u);
-    {
-        uint32_t h[DPT], g[DPT];
-#pragma unroll
-        for (int k = 0; k < DPT; ++k) h[k] = digit_thread ? hist[d0 + k] : 0u;
-        digits_exclusive<G::kDigitWaves, DPT>(h, g, s_w);
-        if (digit_thread)
-#pragma unroll
-            for (int k = 0; k < DPT; ++k) s_global[d0 + k] = g[k];
-    }
-
     for (;;) {
-        uint32_t* zc = reinterpret_cast<uint32_t*>(&s_wave_cnt[0][0]);  // cleared as 32-bit words
+        uint32_t* zc = reinterpret_cast<uint32_t*>(&L.s_wave_cnt[0][0]);  // cleared as 32-bit words
         for (int i = tid; i < (int)(WAVES * R * sizeof(CntT) / 4); i += THREADS) zc[i] = 0;
         __syncthreads();
-        const uint32_t tile = s_tile;
+        const uint32_t tile = L.s_tile;
         if (tile >= num_tiles) break;
-        const uint32_t wbase = tile * TILE + wave * (64 * ITEMS);
-        K key[ITEMS];
-        uint32_t val[ITEMS];
-        uint32_t rank[ITEMS];
+        const uint32_t wbase = tile * tile_keys + wave * (64 * IT);
+        const uint32_t wend = min(n, wbase + 64 * IT);  // this wave's keys: [wbase, wend)
+        K key[IT];
+        uint32_t val[IT];
+        uint32_t rank[IT];
 #pragma unroll
-        for (int i = 0; i < ITEMS; ++i) {
+        for (int i = 0; i < IT; ++i) {
             const uint32_t p = wbase + i * 64 + lane;
-            const bool ok = p < n;
+            const bool ok = p < wend;
             key[i] = ok ? kin[p] : (K)0;
             val[i] = ok ? vin[p] : 0u;
         }
         // Stable per-wave ranks: items in position order, lanes in lane order.
 #pragma unroll
-        for (int i = 0; i < ITEMS; ++i) {
+        for (int i = 0; i < IT; ++i) {
             const uint32_t p = wbase + i * 64 + lane;
             const bool ok = p < n;
             const uint32_t d = (uint32_t)(key[i] >> shift) & mask;
@@ -274,9 +289,9 @@ __global__ __launch_bounds__(THREADS) void k_onesweep(
             const uint64_t m = (uint64_t)m_hi << 32 | m_lo;
             const uint32_t below = count_below(m);
             uint32_t prior = 0;
-            if (ok) prior = s_wave_cnt[wave][d];
+            if (ok) prior = L.s_wave_cnt[wave][d];
             rank[i] = prior + below;
-            if (ok && below == 0) s_wave_cnt[wave][d] = (CntT)(prior + (uint32_t)__popcll(m));
+            if (ok && below == 0) L.s_wave_cnt[wave][d] = (CntT)(prior + (uint32_t)__popcll(m));
         }
         __syncthreads();
         // Per digit: wave prefixes, tile count, publication, look-back.
@@ -288,24 +303,24 @@ __global__ __launch_bounds__(THREADS) void k_onesweep(
             if (digit_thread) {
 #pragma unroll
                 for (int w = 0; w < WAVES; ++w) {
-                    const uint32_t c = s_wave_cnt[w][d0 + k];
-                    s_wave_cnt[w][d0 + k] = (CntT)cnt[k];
+                    const uint32_t c = L.s_wave_cnt[w][d0 + k];
+                    L.s_wave_cnt[w][d0 + k] = (CntT)cnt[k];
                     cnt[k] += c;
                 }
                 st_release_u32(my_status + k, (tile == 0 ? kInc : kAgg) | cnt[k]);
             }
         }
         uint32_t local[DPT];
-        digits_exclusive<G::kDigitWaves, DPT>(cnt, local, s_w);
+        digits_exclusive<G::kDigitWaves, DPT>(cnt, local, L.s_w);
         if constexpr (H > 1) {
             // H adjacent lanes per digit: lane q of digit d loads predecessors
             // q * LBH + 1 .. (q + 1) * LBH back of each step of H * LBH (one
             // round trip), sums them nearest first up to the first inclusive
-            // word, and the H partial results combine in order.  s_scatter
+            // word, and the H partial results combine in order.  L.s_scatter
             // holds the digits' tile counts until the offsets replace them.
             if (digit_thread) {
-                s_local[d0] = local[0];
-                s_scatter[d0] = cnt[0];
+                L.s_local[d0] = local[0];
+                L.s_scatter[d0] = cnt[0];
             }
             __syncthreads();
             const uint32_t d = tid / H, q = tid % H;
@@ -349,9 +364,9 @@ __global__ __launch_bounds__(THREADS) void k_onesweep(
                     if (found) break;
                     j -= (int64_t)H * LBH;
                 }
-                if (q == 0) st_release_u32(status + (size_t)tile * R + d, kInc | ((ex + s_scatter[d]) & kStatusValueMask));
+                if (q == 0) st_release_u32(status + (size_t)tile * R + d, kInc | ((ex + L.s_scatter[d]) & kStatusValueMask));
             }
-            if (q == 0) s_scatter[d] = s_global[d] + ex - s_local[d];
+            if (q == 0) L.s_scatter[d] = L.s_global[d] + ex - L.s_local[d];
         }
         uint32_t excl[DPT];
 #pragma unroll
@@ -398,40 +413,90 @@ __global__ __launch_bounds__(THREADS) void k_onesweep(
         if (H == 1 && digit_thread) {
 #pragma unroll
             for (int k = 0; k < DPT; ++k) {
-                s_local[d0 + k] = local[k];
-                s_scatter[d0 + k] = s_global[d0 + k] + excl[k] - local[k];
+                L.s_local[d0 + k] = local[k];
+                L.s_scatter[d0 + k] = L.s_global[d0 + k] + excl[k] - local[k];
             }
         }
         __syncthreads();
         // Digit-sorted staging of the tile in LDS.
 #pragma unroll
-        for (int i = 0; i < ITEMS; ++i) {
+        for (int i = 0; i < IT; ++i) {
             const uint32_t p = wbase + i * 64 + lane;
             if (p < n) {
                 const uint32_t d = (uint32_t)(key[i] >> shift) & mask;
-                const uint32_t lp = s_local[d] + s_wave_cnt[wave][d] + rank[i];
-                if (lp < (uint32_t)TILE) {
-                    s_keys[lp] = key[i];
-                    s_vals[lp] = val[i];
+                const uint32_t lp = L.s_local[d] + L.s_wave_cnt[wave][d] + rank[i];
+                if (lp < tile_keys) {
+                    L.s_keys[lp] = key[i];
+                    L.s_vals[lp] = val[i];
                 }
             }
         }
         __syncthreads();
-        const uint32_t tile_n = min((uint32_t)TILE, n - tile * TILE);
+        const uint32_t tile_n = min(tile_keys, n - tile * tile_keys);
         for (uint32_t idx = tid; idx < tile_n; idx += THREADS) {
-            const K k = s_keys[idx];
+            const K k = L.s_keys[idx];
             const uint32_t d = (uint32_t)(k >> shift) & mask;
-            const uint32_t o = s_scatter[d] + idx;
+            const uint32_t o = L.s_scatter[d] + idx;
             if (o < n) {  // always true for a consistent histogram; guards the buffer otherwise
                 kout[o] = k;
-                vout[o] = s_vals[idx];
+                vout[o] = L.s_vals[idx];
             }
         }
         // The next ticket is taken only now: a ticket held while this tile is
         // still in flight would hold up that tile's successors.
-        if (tid == 0) s_tile = atomicAdd(ticket, 1u);
+        if (tid == 0) L.s_tile = atomicAdd(ticket, 1u);
         __syncthreads();
     }
+}
+
+template <typename K, int THREADS, int RB>
+__global__ __launch_bounds__(THREADS) void k_onesweep(
+    const K* __restrict__ kin, const uint32_t* __restrict__ vin, K* __restrict__ kout,
+    uint32_t* __restrict__ vout, const uint32_t* n_dev, uint64_t n_host, uint32_t shift, uint32_t bits,
+    const uint32_t* __restrict__ hist, uint32_t* ticket, uint32_t* status, uint32_t* clear_next,
+    uint32_t* error, uint32_t slots) {
+    using G = SortGeom<K, THREADS, RB>;
+    constexpr int ITEMS = G::kItems, R = G::kRadixN, DPT = G::kDPT;
+    static_assert(G::kH == 1 || DPT == 1, "shared look-back needs one digit per owner thread");
+    __shared__ OnesweepLds<K, THREADS, RB> L;
+
+    const uint32_t tid = threadIdx.x;
+    const bool digit_thread = tid < (uint32_t)G::kDigitThreads;
+    const uint32_t d0 = tid * DPT;  // first digit of this thread
+    const uint32_t n = read_n(n_dev, n_host);
+    const uint32_t items = onesweep_items(n, slots, THREADS, ITEMS);  // uniform
+    const uint32_t tile_keys = items * THREADS;
+    const uint32_t num_tiles = ceil_div(n, tile_keys);
+    const uint32_t mask = (1u << bits) - 1u;
+    if (clear_next) {
+        const size_t words = (size_t)num_tiles * R, stride = (size_t)gridDim.x * THREADS;
+        for (size_t i = (size_t)blockIdx.x * THREADS + tid; i < words; i += stride) clear_next[i] = 0;
+    }
+    if (blockIdx.x >= num_tiles) return;  // uniform per block
+    if (tid == 0) L.s_tile = atomicAdd(ticket, 1u);
+    {
+        uint32_t h[DPT], g[DPT];
+#pragma unroll
+        for (int k = 0; k < DPT; ++k) h[k] = digit_thread ? hist[d0 + k] : 0u;
+        digits_exclusive<G::kDigitWaves, DPT>(h, g, L.s_w);
+        if (digit_thread)
+#pragma unroll
+            for (int k = 0; k < DPT; ++k) L.s_global[d0 + k] = g[k];
+    }
+
+    // one instantiation per tile length onesweep_items can pick
+    constexpr int I1 = onesweep_item_choice(ITEMS, 1), I2 = onesweep_item_choice(ITEMS, 2),
+                  I3 = onesweep_item_choice(ITEMS, 3), I4 = onesweep_item_choice(ITEMS, 4);
+    if (items == (uint32_t)ITEMS)
+        onesweep_tiles<K, THREADS, RB, ITEMS>(L, kin, vin, kout, vout, n, num_tiles, shift, mask, ticket, status, error);
+    else if (items == (uint32_t)I1)
+        onesweep_tiles<K, THREADS, RB, I1>(L, kin, vin, kout, vout, n, num_tiles, shift, mask, ticket, status, error);
+    else if (items == (uint32_t)I2)
+        onesweep_tiles<K, THREADS, RB, I2>(L, kin, vin, kout, vout, n, num_tiles, shift, mask, ticket, status, error);
+    else if (items == (uint32_t)I3)
+        onesweep_tiles<K, THREADS, RB, I3>(L, kin, vin, kout, vout, n, num_tiles, shift, mask, ticket, status, error);
+    else
+        onesweep_tiles<K, THREADS, RB, I4>(L, kin, vin, kout, vout, n, num_tiles, shift, mask, ticket, status, error);
 }
 
 template <typename K, int THREADS, int RB>
@@ -444,7 +509,11 @@ int sort_impl(Context* c, K* k0, uint32_t* v0, K* k1, uint32_t* v1, const uint32
     *result_index = (int)(passes & 1u);
     if (passes == 0) return GSR_OK;
     if (passes > (uint32_t)kMaxPasses) return fail(c, GSR_ERR_ARGUMENT, "too many radix passes (%u)", passes);
-    const uint32_t max_tiles = std::max<uint32_t>(1, ceil_div(capacity, (uint64_t)G::kTile));
+    // resident onesweep blocks; the tile is shortened to give each one tile
+    // while the keys do not fill them all (onesweep_items), so a pass has at
+    // most max(slots, full tiles of the capacity) tiles
+    const uint32_t slots = (uint32_t)c->cu_count * (1024 / THREADS);
+    const uint32_t max_tiles = std::max<uint32_t>(slots, ceil_div(capacity, (uint64_t)G::kTile));
     const size_t hist_words = (size_t)passes * R + kMaxPasses;
     uint32_t* hist = (uint32_t*)scratch(c, "sort_hist", hist_words * 4);
     // two sets of look-back words, alternating between passes
@@ -460,17 +529,17 @@ int sort_impl(Context* c, K* k0, uint32_t* v0, K* k1, uint32_t* v1, const uint32
     const uint32_t hist_blocks = std::min<uint32_t>(
         std::max<uint32_t>(1, ceil_div(capacity, (uint64_t)1024 * kHistItems)), (uint32_t)c->cu_count);
     hipLaunchKernelGGL((k_sort_hist<K, RB>), dim3(hist_blocks), dim3(1024), 0, c->stream, k0, n_dev, n_host,
-                       begin_bit, end_bit, passes, hist, status, (uint32_t)G::kTile);
-    const uint32_t sweep_blocks = std::min<uint32_t>(max_tiles, (uint32_t)c->cu_count * (1024 / THREADS));
+                       begin_bit, end_bit, passes, hist, status, slots, (uint32_t)THREADS, (uint32_t)G::kItems);
     K* kb[2] = {k0, k1};
     uint32_t* vb[2] = {v0, v1};
     uint32_t* st[2] = {status, status + (size_t)max_tiles * R};
     for (uint32_t p = 0; p < passes; ++p) {
         const uint32_t shift = begin_bit + p * RB;
         const uint32_t bits = std::min<uint32_t>(RB, end_bit - shift);
-        hipLaunchKernelGGL((k_onesweep<K, THREADS, RB>), dim3(sweep_blocks), dim3(THREADS), 0, c->stream, kb[p & 1],
+        hipLaunchKernelGGL((k_onesweep<K, THREADS, RB>), dim3(slots), dim3(THREADS), 0, c->stream, kb[p & 1],
                            vb[p & 1], kb[(p + 1) & 1], vb[(p + 1) & 1], n_dev, n_host, shift, bits, hist + p * R,
-                           tickets + p, st[p & 1], p + 1 < passes ? st[(p + 1) & 1] : nullptr, &counters->error);
+                           tickets + p, st[p & 1], p + 1 < passes ? st[(p + 1) & 1] : nullptr, &counters->error,
+                           slots);
     }
     GSR_HIP(c, hipGetLastError());
     return GSR_OK;
