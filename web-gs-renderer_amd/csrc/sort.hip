@@ -44,12 +44,16 @@ namespace {
 constexpr uint32_t kAgg = 1u << 30, kInc = 2u << 30;
 constexpr int kLookBack = 16;   // predecessor words fetched per look-back step (all digits of a thread)
 constexpr int kHistItems = 16;  // keys per thread per histogram block tile
+// onesweep keys per thread at most for 64-bit keys in 1024-thread blocks
+// (8 in smaller blocks, whose several blocks per CU need fewer registers)
+constexpr int kMaxItems64 = 12;
 constexpr size_t kLdsBytes = 160 * 1024;
 
 // Onesweep geometry for keys K, THREADS threads per block and RB-bit digits.
 // Each thread ranks up to kItems keys; the full tile is THREADS * kItems
-// keys, the largest of 16 / 8 / 4 items whose LDS (keys and values staged, 16-bit
-// per-wave digit counts, three digit tables) fits one CU.  Digits are owned
+// keys, the most items (up to 16, fewer for 64-bit keys) whose
+// LDS (keys and values staged, 16-bit per-wave digit counts, three digit
+// tables) fits one CU.  Digits are owned
 // kDPT per thread (thread t: digits t * kDPT ...), so 2048 digits need no
 // more than 512 threads.
 template <typename K, int THREADS, int RB>
@@ -66,9 +70,8 @@ struct SortGeom {
         return (size_t)THREADS * items * (sizeof(K) + 4) + (size_t)kWaves * kRadixN * sizeof(CntT) + (size_t)3 * kRadixN * 4 +
                64 * 4 + 64;
     }
-    static constexpr int kItems = lds(sizeof(K) == 4 ? 16 : 8) <= kLdsBytes ? (sizeof(K) == 4 ? 16 : 8)
-                                  : lds(sizeof(K) == 4 ? 8 : 4) <= kLdsBytes ? (sizeof(K) == 4 ? 8 : 4)
-                                                                              : (sizeof(K) == 4 ? 4 : 2);
+    static constexpr int fit(int items) { return items <= 1 || lds(items) <= kLdsBytes ? items : fit(items - 1); }
+    static constexpr int kItems = fit(sizeof(K) == 4 ? 16 : THREADS == 1024 ? kMaxItems64 : 8);
     static constexpr int kTile = THREADS * kItems;
     static_assert(lds(kItems) <= kLdsBytes, "onesweep geometry does not fit LDS");
     static constexpr int kLB = kLookBack / kDPT > 4 ? kLookBack / kDPT : 4;  // predecessors per step
@@ -591,8 +594,8 @@ int sort_digits(Context* c, K* k0, uint32_t* v0, K* k1, uint32_t* v1, const uint
 
 uint32_t default_digit_bits(uint32_t key_bytes, uint32_t end_bit) {
     if (key_bytes != 8) return 8;  // wider digits would halve the 32-bit key tile (LDS)
-    // 64-bit keys keep 8192-key tiles up to 10-bit digits: the narrowest
-    // width with the fewest passes
+    // 64-bit keys keep 9216-key or longer tiles up to 10-bit digits: the
+    // narrowest width with the fewest passes
     uint32_t best = 8;
     for (uint32_t rb = 9; rb <= 10; ++rb)
         if (sort_passes(0, end_bit, rb) < sort_passes(0, end_bit, best)) best = rb;
