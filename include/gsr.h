@@ -63,7 +63,8 @@ typedef struct gsr_options {
     uint32_t tile_size;        /* 8, 16 or 32 pixels; 0 = 16 (the reference's 16x16)            */
     uint32_t key_bytes;        /* GSR_KEY_AUTO / GSR_KEY_32 / GSR_KEY_64                          */
     uint64_t instance_capacity;/* tile-instance slots; 0 = 64 * splat count (renderer.js:16,536) */
-    uint32_t timing;           /* 1 = record hipEvents around every stage (gsr_get_timings)      */
+    uint32_t timing;           /* 1 = hipEvents around every stage (gsr_get_timings);            */
+                               /* 2 = around the raster stages only (two events per raster)     */
     uint32_t save_state;       /* 1 = keep per-pixel T_final / last contributor for backward     */
     uint32_t raster_waves_per_cu; /* batches: persistent raster waves per CU; 0 = 16              */
     uint32_t sort_blocks_per_cu;  /* onesweep blocks per CU: 1, 2 or 4 (1024-, 512-, 256-thread); 0 = 1 */
@@ -125,7 +126,8 @@ int gsr_forward(gsr_context* ctx, const float* params_dev, uint32_t count,
 /* Number of tile instances (raw scan total I) of the last forward; syncs. */
 int gsr_last_instance_count(gsr_context* ctx, uint64_t* instances);
 
-/* Per-pass GPU times of the last frame(s) with opts.timing = 1 (profiler.js:52-123). */
+/* Per-pass GPU times of the last frame(s) with opts.timing = 1, or of the raster passes with
+   opts.timing = 2 (profiler.js:52-123). */
 int gsr_get_timings(gsr_context* ctx, gsr_timing_row* rows, int capacity, int* count);
 
 /* rgba8 conversion with WebGPU unorm8 semantics (raseter.wgsl:19,109): clamp to

@@ -36,6 +36,12 @@ def test_render_small_matches_oracle(torch_cuda, ctx, orc):
     rows = ctx.timings()
     assert {"preprocess", "offset scan", "emit", "sort", "tile ranges", "raster", "TOTAL"} <= set(rows)
     assert rows["TOTAL"] > 0
+    # timing mode 2: events around the raster only, same image
+    img2 = ctx.render(u, lib.options(timing=2))
+    assert np.array_equal(img2, img)
+    rows2 = ctx.timings()
+    assert set(rows2) == {"raster", "TOTAL"}
+    assert rows2["raster"] > 0
 
 
 def test_forward_from_device_params(torch_cuda, ctx, orc):

@@ -211,8 +211,8 @@ static void drop_graph(Context* c) {
 
 static int capture_batch(Context* c, const float* params, uint32_t count, const BatchPlan& p, float* out) {
     drop_graph(c);
-    const bool timing = c->timing_on;
-    c->timing_on = false;
+    const uint32_t timing = c->timing_on;
+    c->timing_on = 0;
     hipGraph_t graph = nullptr;
     GSR_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     int rc = enqueue_batch(c, params, count, p, out);
@@ -324,7 +324,7 @@ int gsr_render_batch(gsr_context* ctx, const float* params_dev, uint32_t count, 
     if (!c) return fail(nullptr, GSR_ERR_ARGUMENT, "null context");
     if (!params_dev && count > 0) return fail(c, GSR_ERR_ARGUMENT, "gsr_render_batch: no parameters");
     hipSetDevice(c->device);
-    c->timing_on = opts && opts->timing;
+    c->timing_on = opts ? opts->timing : 0;
     return run_batch(c, params_dev, count, cams, views, opts, use_graph, out_rgb_dev, out_offsets);
 }
 
@@ -352,8 +352,8 @@ int gsr_autotune_batch(gsr_context* ctx, const float* params_dev, uint32_t count
     GSR_HIP(c, hipEventCreate(&e1));
     double best = 1e30;
     gsr_options best_opts = *inout;
-    const bool timing = c->timing_on;
-    c->timing_on = false;
+    const uint32_t timing = c->timing_on;
+    c->timing_on = 0;
     int rc = GSR_OK;
     for (uint32_t ti = 0; ti < 3 && rc == GSR_OK; ++ti)
         for (uint32_t wi = 0; wi < 3 && rc == GSR_OK; ++wi)

@@ -163,7 +163,7 @@ struct Context {
     TimingAcc timing;
     std::vector<gsr_timing_row> last_rows;
     int last_frames = 0;
-    bool timing_on = false;
+    uint32_t timing_on = 0;  // gsr_options.timing of the current call
     // saved forward state for backward
     bool have_state = false;
     uint32_t state_count = 0, state_w = 0, state_h = 0, state_tile = 16, state_key_bytes = 4;

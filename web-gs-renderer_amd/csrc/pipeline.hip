@@ -186,7 +186,7 @@ int gsr_forward(gsr_context* ctx, const float* params_dev, uint32_t count, const
     FramePlan p;
     int rc = plan_frame(c, count, cam, opts, &p);
     if (rc) return rc;
-    c->timing_on = opts && opts->timing;
+    c->timing_on = opts ? opts->timing : 0;
     if (c->marks.size() > 4096) {  // bound the event backlog of long timed runs
         GSR_HIP(c, hipStreamSynchronize(c->stream));
         finish_frame_timings(c);
@@ -219,7 +219,7 @@ int gsr_render(gsr_context* ctx, const gsr_camera* cam, const gsr_options* opts,
     FramePlan p;
     int rc = plan_frame(c, it->second.count, cam, opts, &p);
     if (rc) return rc;
-    c->timing_on = opts && opts->timing;
+    c->timing_on = opts ? opts->timing : 0;
     rc = enqueue_forward(c, it->second.params, it->second.count, cam, p, opts && opts->save_state, nullptr);
     if (rc) return rc;
     const size_t bytes = (size_t)p.W * p.H * 12;

@@ -124,7 +124,7 @@ def last_error(ctx=None) -> str:
 
 def options(tile_size=16, key_bytes=KEY_AUTO, capacity=0, timing=False, save_state=False, raster_waves_per_cu=0,
             sort_blocks_per_cu=0, sort_digit_bits=0) -> Options:
-    return Options(tile_size, key_bytes, capacity, 1 if timing else 0, 1 if save_state else 0, raster_waves_per_cu,
+    return Options(tile_size, key_bytes, capacity, int(timing), 1 if save_state else 0, raster_waves_per_cu,
                    sort_blocks_per_cu, sort_digit_bits)
 
 
