@@ -29,7 +29,7 @@ FP32_PEAK_TFLOPS = 157.3
 # HBM bytes per launch of the forward's kernels at this workload, from the
 # rocprofv3 FETCH_SIZE / WRITE_SIZE runs summarised in
 # profiles/forward_1m_1080p_pmc_traffic.md (calibrated against a plain copy).
-MEASURED_TRAFFIC = {"raster": 620154644, "preprocess": 304120839, "sort": 240074584}
+MEASURED_TRAFFIC = {"raster": 623762504, "preprocess": 304208896, "sort": 242396176}
 
 
 def log(msg):
