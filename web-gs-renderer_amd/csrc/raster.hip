@@ -26,19 +26,22 @@
 namespace gsr {
 namespace {
 
-// Blocks take 4096-key chunks in a grid-stride loop; each wave loads 16
-// rows of 64 consecutive keys (coalesced, all in flight), and a key's
+// Blocks take kRangeChunk-key chunks in a grid-stride loop; each wave loads
+// kRangeItems rows of 64 consecutive keys (coalesced, all in flight), and a key's
 // predecessor comes from the neighbouring lane (DPP wave_shr:1), from lane 63
 // of the previous row, or, for the wave's first key, from memory.
+constexpr int kRangeItems = 8;  // 8 rows and 8 blocks per CU: 2.7 us less than 16 and 4 at 1080p
+constexpr uint32_t kRangeChunk = 4 * 64 * kRangeItems;
+
 template <typename K>
 __global__ __launch_bounds__(256) void k_ranges(const K* __restrict__ keys, const uint32_t* n_dev,
                                                 uint64_t n_host, uint32_t num_tiles,
                                                 uint32_t* __restrict__ ranges) {
-    constexpr int ITEMS = 16;
+    constexpr int ITEMS = kRangeItems;
     const uint32_t n = n_dev ? *n_dev : (uint32_t)n_host;
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (uint64_t chunk = blockIdx.x; chunk * 4096 < n; chunk += gridDim.x) {
-        const uint64_t wbase = chunk * 4096 + wave * (64 * ITEMS);
+    for (uint64_t chunk = blockIdx.x; chunk * kRangeChunk < n; chunk += gridDim.x) {
+        const uint64_t wbase = chunk * kRangeChunk + wave * (64 * ITEMS);
         uint32_t tile[ITEMS];
 #pragma unroll
         for (int i = 0; i < ITEMS; ++i) {
@@ -320,8 +323,8 @@ int launch_ranges(Context* c, const void* keys, uint32_t key_bytes, const uint32
         int rc = launch_fill_u32(c, ranges, 0u, (size_t)num_tiles * 2);
         if (rc) return rc;
     }
-    uint32_t blocks = (uint32_t)c->cu_count * 4;  // device count: sized for any n
-    if (!n_dev) blocks = (uint32_t)std::min<uint64_t>(blocks, std::max<uint64_t>(1, ceil_div(n_host, (uint64_t)4096)));
+    uint32_t blocks = (uint32_t)c->cu_count * 8;  // device count: sized for any n
+    if (!n_dev) blocks = (uint32_t)std::min<uint64_t>(blocks, std::max<uint64_t>(1, ceil_div(n_host, (uint64_t)kRangeChunk)));
     if (key_bytes == 8)
         hipLaunchKernelGGL(k_ranges<uint64_t>, dim3(blocks), dim3(256), 0, c->stream, (const uint64_t*)keys,
                            n_dev, n_host, num_tiles, ranges);
