@@ -6,7 +6,7 @@ N(-5, 0.7), SH degree 3), camera at (0,0,-8) looking at the origin, 50 deg
 vertical FOV, 1920x1080, 16x16 tiles, seed 1.  Synthetic data.
 
 Prints one JSON line.  `roofline` describes the raster, the stage with the
-largest share of the frame (checked against a per-stage run), timed with HIP
+largest share of the frame (reported from a per-stage run), timed with HIP
 events around it on the renderer's stream over the timed region (events
 around every stage would add gaps of their own to the frames timed);
 `stage_ms` comes from a separate run with events around every stage.
