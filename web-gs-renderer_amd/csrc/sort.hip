@@ -218,9 +218,6 @@ __global__ __launch_bounds__(1024) void k_sort_hist(const K* __restrict__ keys, 
     }
 }
 
-// One pass.  status: this pass's look-back words (tile * R + digit), zero
-// for every tile of the key count; clear_next: the next pass's words, which
-// this pass zeroes (null on the last pass).
 // LDS of a onesweep block (one allocation whatever the tile length).
 template <typename K, int THREADS, int RB>
 struct OnesweepLds {
@@ -452,6 +449,9 @@ __device__ __forceinline__ void onesweep_tiles(OnesweepLds<K, THREADS, RB>& L, c
     }
 }
 
+// One pass.  status: this pass's look-back words (tile * R + digit), zero
+// for every tile of the key count; clear_next: the next pass's words, which
+// this pass zeroes (null on the last pass).
 template <typename K, int THREADS, int RB>
 __global__ __launch_bounds__(THREADS) void k_onesweep(
     const K* __restrict__ kin, const uint32_t* __restrict__ vin, K* __restrict__ kout,
