@@ -231,6 +231,24 @@ def test_sort_emitted_instances_exact(torch_cuda, ctx, orc):
         assert np.array_equal(k, ks) and np.array_equal(v, vs)
 
 
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 2047, 2048, 2049, 4095, 4096, 4097, 100_003])
+def test_tile_ranges_chunk_edges(torch_cuda, ctx, orc, n):
+    """Key counts at the range kernel's row (64) and chunk (2048) edges, with
+    tile runs crossing them and empty tiles between runs."""
+    rng = np.random.default_rng(n)
+    num_tiles = 300
+    tiles = np.sort(rng.integers(0, num_tiles, size=n)).astype(np.uint64)
+    ks = (tiles << np.uint64(16)) | rng.integers(0, 1 << 16, size=n).astype(np.uint64)
+    ks.sort()
+    ref = orc.tile_ranges(ks, num_tiles)
+    for kb, kdt in ((4, np.uint32), (8, np.uint64)):
+        d_k = to_dev(torch_cuda, ks.astype(kdt))
+        d_r = to_dev(torch_cuda, np.full(2 * num_tiles, 7, np.uint32))
+        torch_cuda.cuda.synchronize()
+        ctx.tile_ranges(d_k, kb, n, num_tiles, d_r)
+        assert np.array_equal(to_host(d_r, np.uint32), ref), (n, kb)
+
+
 @pytest.mark.parametrize("key_bytes", [4, 8])
 def test_tile_ranges_exact(torch_cuda, ctx, orc, key_bytes):
     sc, cam = scenes.workload_small()
