@@ -26,7 +26,9 @@
 // formulas) measure the FP32 error.
 //
 // Pins (tests/test_oracle.py): exp_f32 against libm within 2 ulp, exp_blend_f32
-// within 2.7 ulp on [-5.55, 0]; the integer
+// within 2.7 ulp on [-5.55, 0]; the FP32 blend decision (blend_range_f32 and
+// the alpha tests) against FP64 libm over every power from 0 down to -inf,
+// and a raster of far thin splats against the FP64 raster; the integer
 // chain against closed forms / brute force; preprocess special cases
 // (isotropic splat, on-axis Jacobian, SH degree 0, clamp, sigmoid(0));
 // FP32 vs FP64 agreement; raster closed forms on 1-3 splats.
@@ -102,15 +104,30 @@ inline float exp_blend_f32(float p) {
     return ldexpf(q, (int32_t)(tu - ru));
 }
 
+// Which powers the FP32 blend evaluates.  raseter.wgsl:84-92 skips a splat
+// at a pixel when power > 0, and when alpha = min(0.99, op e^power) < 1/255.
+// The opacity is a sigmoid (op <= 1) and e^-5.55 = 0.003887 < 1/255 =
+// 0.003922, so every power below -5.55 is skipped by the alpha test as well.
+// The FP32 paths (raster, the FP32 decision replay of the adjoint, the FP32
+// backward twin) drop those powers before the exponential, exactly as the
+// GPU kernels do: exp_blend_f32 is accurate on [-5.55, 0] only, and below
+// about -8.7e6 its 2^k scale wraps (inf, NaN at -inf).  A NaN power (from a
+// non-finite conic) passes both tests, as in the kernels.  The FP64 paths
+// keep the reference's tests as written (libm exp is exact enough there),
+// which is what pins this cut (tests/test_oracle.py).
+inline bool blend_range_f32(float power) { return !(power > 0.0f) && !(power < -5.55f); }
+
 template <typename T> struct M;
 template <> struct M<float> {
     static float ex(float x) { return exp_f32(x); }
     static float ex_blend(float x) { return exp_blend_f32(x); }
+    static bool in_blend_range(float p) { return blend_range_f32(p); }
     static float fma(float a, float b, float c) { return fmaf(a, b, c); }
 };
 template <> struct M<double> {
     static double ex(double x) { return std::exp(x); }
     static double ex_blend(double x) { return std::exp(x); }
+    static bool in_blend_range(double p) { return !(p > 0.0); }
     static double fma(double a, double b, double c) { return std::fma(a, b, c); }
 };
 
@@ -455,7 +472,7 @@ void raster_all(uint32_t W, uint32_t H, uint32_t tile, const uint32_t* ranges, c
                     // power = -0.5 (A dx^2 + C dy^2) - B dx dy  (calculatePower :113-117)
                     const T q = M<T>::fma(A * dx, dx, (C * dy) * dy);
                     const T power = M<T>::fma(T(-0.5), q, -((B * dx) * dy));
-                    if (power > T(0)) continue;
+                    if (!M<T>::in_blend_range(power)) continue;
                     const T alpha = std::min(T(0.99f), op * M<T>::ex_blend(power));
                     if (alpha < T(1.0f / 255.0f)) continue;
                     const T testT = Tr * (T(1) - alpha);
@@ -791,7 +808,7 @@ void backward_f64(const float* packed, uint32_t n, const Cam& cam, uint32_t tile
                             const float dx = xyf[g * 2] - (float)x, dy = xyf[g * 2 + 1] - (float)y;
                             const float q = fmaf(A * dx, dx, (C * dy) * dy);
                             const float power = fmaf(-0.5f, q, -((B * dx) * dy));
-                            if (power > 0.0f) continue;
+                            if (!blend_range_f32(power)) continue;
                             const float oe = conicf[g * 4 + 3] * exp_blend_f32(power);
                             const float alpha = std::min(0.99f, oe);
                             if (alpha < 1.0f / 255.0f) continue;
@@ -932,7 +949,7 @@ void raster_backward_f32(uint32_t W, uint32_t H, uint32_t tile, const uint32_t* 
                     const float dx = xy[g * 2] - (float)x, dy = xy[g * 2 + 1] - (float)y;
                     const float q = fmaf(A * dx, dx, (C * dy) * dy);
                     const float power = fmaf(-0.5f, q, -((B * dx) * dy));
-                    if (power > 0.0f || power < -5.55f) continue;
+                    if (!blend_range_f32(power)) continue;
                     const float e = exp_blend_f32(power);
                     const float oe = op * e;
                     const float alpha = std::min(0.99f, oe);
@@ -982,6 +999,20 @@ float orc_exp_f32(float x) { return exp_f32(x); }
 
 void orc_exp_blend_f32_array(const float* x, float* y, int64_t n) {
     for (int64_t i = 0; i < n; ++i) y[i] = exp_blend_f32(x[i]);
+}
+
+// The FP32 blend's alpha for (power, opacity) pairs, 0 where the pixel skips
+// the splat (the raster's tests in its order, blend_range_f32 first).
+void orc_blend_alpha_f32(const float* power, const float* op, float* alpha, int64_t n) {
+    for (int64_t i = 0; i < n; ++i) {
+        const float p = power[i];
+        float a = 0.0f;
+        if (blend_range_f32(p)) {
+            a = std::min(0.99f, op[i] * exp_blend_f32(p));
+            if (a < 1.0f / 255.0f) a = 0.0f;
+        }
+        alpha[i] = a;
+    }
 }
 
 void orc_exp_f32_array(const float* x, float* y, int64_t n) {

@@ -35,6 +35,7 @@ def lib():
         L.orc_exp_f32.argtypes = [ctypes.c_float]
         L.orc_exp_f32_array.argtypes = [_f32p, _f32p, ctypes.c_int64]
         L.orc_exp_blend_f32_array.argtypes = [_f32p, _f32p, ctypes.c_int64]
+        L.orc_blend_alpha_f32.argtypes = [_f32p, _f32p, _f32p, ctypes.c_int64]
         L.orc_num_threads.restype = ctypes.c_int
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_preprocess_f32.argtypes = [_f32p, ctypes.c_uint32, _f32p, ctypes.c_uint32,
@@ -82,6 +83,16 @@ def exp_blend_f32(x):
     y = np.empty_like(x)
     lib().orc_exp_blend_f32_array(x.ravel(), y.ravel(), x.size)
     return y
+
+
+def blend_alpha_f32(power, op):
+    """The FP32 raster's alpha for each (power, opacity) pair, 0 where the pixel skips the splat."""
+    p, o = np.broadcast_arrays(np.asarray(power, np.float32), np.asarray(op, np.float32))
+    p = np.ascontiguousarray(p, np.float32)
+    o = np.ascontiguousarray(o, np.float32)
+    a = np.empty_like(p)
+    lib().orc_blend_alpha_f32(p.ravel(), o.ravel(), a.ravel(), p.size)
+    return a
 
 
 def preprocess(packed, n, uni, tile=16, fp64=False):

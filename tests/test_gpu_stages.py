@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 from conftest import empty_dev, to_dev, to_host
+from edge_scenes import edge_scene as _edge_scene
 from gsr import lib, scenes
 
 pytestmark = pytest.mark.gpu
@@ -33,22 +34,6 @@ def _pre_gpu(torch, ctx, sc, u, tile=16):
     return dict(tiles=to_host(tiles, np.uint32)[:n], meta=to_host(meta, np.uint32)[:4 * n].reshape(n, 4),
                 conic=to_host(conic, np.float32)[:4 * n].reshape(n, 4), xy=to_host(xy, np.float32)[:2 * n].reshape(n, 2),
                 rgb=to_host(rgb, np.float32)[:4 * n].reshape(n, 4))
-
-
-def _edge_scene(n=4096, seed=11):
-    """Uniform splats plus planted edge cases: near-plane, behind the camera,
-    off-screen, huge and tiny scales, unnormalised and degenerate quaternions."""
-    sc = scenes.uniform_scene(n, seed=seed, extent=2.5, scale_mu=-3.0, scale_sigma=1.5)
-    sc.pos[0] = (0.0, 0.0, -3.85)
-    sc.pos[1] = (0.0, 0.0, -5.0)
-    sc.pos[2] = (40.0, 0.0, 0.0)
-    sc.lscale[3] = (1.0, 1.0, 1.0)
-    sc.lscale[4] = (-12.0, -12.0, -12.0)
-    sc.rot[5] = (3.0, 0.0, 0.0, 0.0)
-    sc.rot[6] = (0.0, 0.0, 0.0, 1e-3)
-    sc.opacity[7] = 30.0
-    sc.opacity[8] = -30.0
-    return sc
 
 
 @pytest.mark.parametrize("which,tile", [("small", 16), ("edge", 16), ("edge", 8), ("edge", 32)])

@@ -48,6 +48,120 @@ def test_exp_blend_matches_libm_on_the_blend_range(orc):
     assert (np.diff(y) >= 0).all()
 
 
+def _blend_alpha_f64(power, op):
+    """raseter.wgsl:84-92 in FP64 with libm exp: skip (0) when power > 0 or alpha < 1/255."""
+    p = np.asarray(power, np.float64)
+    with np.errstate(over="ignore", invalid="ignore"):
+        a = np.minimum(np.float64(np.float32(0.99)), np.asarray(op, np.float64) * np.exp(p))
+    return np.where((p > 0) | (a < np.float64(np.float32(1.0 / 255.0))), 0.0, a)
+
+
+def test_blend_decision_matches_fp64_at_every_power(orc):
+    """The FP32 blend (power range cut, exp_blend_f32, 0.99 clamp, 1/255 test) takes
+    the FP64 decision at every power from 0 down to -inf, including the powers
+    below about -8.7e6 where exp_blend_f32's 2^k scale wraps: those must be
+    skipped, never blended at the clamp."""
+    far = np.array([-1e4, -8.8e6, -1e7, -1e30, -np.inf, -3.4e38], np.float32)
+    sweep = -np.geomspace(1e-7, 3e38, 20001).astype(np.float32)
+    near_cut = np.linspace(-6.5, -4.5, 20001, dtype=np.float32)
+    powers = np.concatenate([far, sweep, near_cut, [0.0, -0.0, 1e-6, 2.0]]).astype(np.float32)
+    ops = np.array([1e-3, 0.00392, 0.004, 0.05, 0.5, 0.9, 0.99, 0.999999, 1.0], np.float32)
+    P, O = np.meshgrid(powers, ops, indexing="ij")
+    a32 = orc.blend_alpha_f32(P, O).astype(np.float64)
+    a64 = _blend_alpha_f64(P.astype(np.float64), O.astype(np.float64))
+    assert np.isfinite(a32).all()
+    # the far powers are skipped for every opacity
+    assert not a32[: far.size].any()
+    # decisions agree except where FP64 alpha is within 4 ulp of the 1/255 threshold
+    thr = np.float64(np.float32(1.0 / 255.0))
+    edge = np.abs(np.minimum(0.99, O.astype(np.float64) * np.exp(P.astype(np.float64))) - thr) <= 4 * np.spacing(np.float32(thr))
+    differ = (a32 > 0) != (a64 > 0)
+    assert not (differ & ~edge).any()
+    assert edge.sum() < 50
+    both = (a32 > 0) & (a64 > 0)
+    ulp = np.spacing(a64[both].astype(np.float32)).astype(np.float64)
+    assert (np.abs(a32[both] - a64[both]) / ulp).max() < 3.5
+    # a NaN power (non-finite conic) passes both range tests and takes the clamp, as in the kernels
+    assert orc.blend_alpha_f32(np.array([np.nan], np.float32), np.float32(0.5))[0] == np.float32(0.99)
+
+
+@pytest.mark.parametrize("dist", [20.0, 300.0, 2500.0, 1e4, 1e6])
+def test_raster_far_thin_splat_matches_fp64(orc, dist):
+    """A thin splat (conic (3, 0, 1e-9), opacity 0.9) whose centre lies `dist`
+    pixels to the side of a 16x16 tile, as a needle splat near the camera with
+    an off-screen centre puts it: the FP32 raster blends exactly the pixels the
+    FP64 raster blends (none once the power is below -5.55 everywhere)."""
+    conic = [[3.0, 0.0, 1e-9, 0.9]]
+    xy = [[-dist, 8.0]]
+    rgb = [[0.3, 0.6, 0.9, 0.0]]
+    img, T, last = _raster_one(orc, conic, xy, rgb)
+    img64, T64, last64 = _raster_one(orc, conic, xy, rgb, fp64=True)
+    assert np.array_equal(last, last64)
+    assert np.abs(T - T64).max() < 1e-6 and np.abs(img - img64).max() < 1e-6
+    if dist >= 20.0:
+        assert not last.any() and (T == 1.0).all() and not img.any()
+
+
+def test_raster_far_splat_mixed_powers(orc):
+    """One 64x64 frame in which one splat (wide in x, very steep in y, centre just
+    right of the frame) meets powers from about -0.2 at its own row down to
+    about -9e6 at the frame's top and bottom rows, across the wrap of the
+    exponential's scale: FP32 and FP64 rasters blend the same pixels."""
+    W = H = 64
+    conic = [[0.01, 0.0, 2e4, 0.95]]
+    rgb = [[1.0, 1.0, 1.0, 0.0]]
+    img, T, last = _raster_one(orc, conic, [[70.0, 30.0]], rgb, W=W, H=H)
+    img64, T64, last64 = _raster_one(orc, conic, [[70.0, 30.0]], rgb, W=W, H=H, fp64=True)
+    assert np.array_equal(last, last64) and last[30].any() and not last[0].any() and not last[63].any()
+    assert np.abs(T - T64).max() < 1e-6
+    for dist in (3e3, 4e4):
+        img, T, last = _raster_one(orc, conic, [[-dist, 30.0]], rgb, W=W, H=H)
+        assert not last.any() and (T == 1.0).all()
+
+
+def _subset(sc, keep):
+    return scenes.Scene(pos=sc.pos[keep], lscale=sc.lscale[keep], rot=sc.rot[keep], dc=sc.dc[keep],
+                        opacity=sc.opacity[keep], rest=sc.rest[keep])
+
+
+def test_edge_frame_far_needles_blend_nowhere(orc):
+    """The edge frame's far needle splats cover every tile of the frame and meet
+    every pixel at powers below -8.7e6 (the class the blend's exponential cannot
+    evaluate): removing them leaves the FP32 and the FP64 frames bit for bit as
+    they were, with the instance count lower by exactly their tiles."""
+    from edge_scenes import edge_frame, powers_in_rect
+    sc, cam = edge_frame()
+    u = cam.uniform(sc.count)
+    pre = orc.preprocess(sc.packed(), sc.count, u)
+    far = [g for g in range(4096, sc.count)
+           if pre["tiles"][g] > 0 and np.nanmax(powers_in_rect(pre, g, cam.width, cam.height)) < -8.7e6]
+    assert len(far) == 15 and all(pre["tiles"][g] == 19 * 13 for g in far)
+    keep = np.setdiff1d(np.arange(sc.count), far)
+    sub = _subset(sc, keep)
+    for fp64 in (False, True):
+        a, Ta, _, Ia, _ = orc.render(sc.packed(), sc.count, u, fp64=fp64)
+        b, Tb, _, Ib, _ = orc.render(sub.packed(), sub.count, cam.uniform(sub.count), fp64=fp64)
+        assert Ia == Ib + 15 * 19 * 13
+        assert np.array_equal(a, b) and np.array_equal(Ta, Tb), fp64
+
+
+def test_edge_frame_fp32_vs_fp64(orc):
+    """The whole edge frame (near-plane, behind-camera, off-screen, degenerate
+    and huge splats, near needles): FP32 and FP64 frames take the same tile
+    decisions (instance count) and the FP32 image stays within the error the
+    FP32 arithmetic explains (a few cliff pixels where alpha or T lands on a
+    threshold)."""
+    from edge_scenes import edge_frame
+    sc, cam = edge_frame()
+    u = cam.uniform(sc.count)
+    a, Ta, la, Ia, _ = orc.render(sc.packed(), sc.count, u)
+    b, Tb, lb, Ib, _ = orc.render(sc.packed(), sc.count, u, fp64=True)
+    assert Ia == Ib
+    assert np.isfinite(a).all() and np.isfinite(Ta).all()
+    d = np.abs(a - b)
+    assert d.mean() < 3e-5 and d.max() < 0.05 and (la != lb).mean() < 0.02
+
+
 @pytest.mark.parametrize("n", [0, 1, 7, 4096, 100003])
 def test_scan_is_exclusive_prefix_sum(orc, n):
     rng = np.random.default_rng(n)
@@ -191,7 +305,8 @@ def test_preprocess_fp32_vs_fp64(orc):
 def _raster_one(orc, conic, xy, rgb, W=16, H=16, fp64=False):
     n = len(xy)
     vals = np.arange(n, dtype=np.uint32)
-    ranges = np.array([0, n], np.uint32)
+    tiles = ((W + 15) // 16) * ((H + 15) // 16)
+    ranges = np.tile(np.array([0, n], np.uint32), tiles)   # every tile holds every splat
     return orc.raster(W, H, 16, ranges, vals, np.array(conic), np.array(xy), np.array(rgb), fp64=fp64)
 
 
