@@ -87,6 +87,10 @@ void        gsr_destroy(gsr_context* ctx);
 /* Message of the last failure on ctx; with ctx == NULL, of the last failed
  * gsr_create on this thread.  Never NULL. */
 const char* gsr_last_error(const gsr_context* ctx);
+/* Waits for the context's stream.  When a gsr_forward ran since its counters
+ * were last read, also reads them and reports what gsr_render would:
+ * GSR_ERR_CAPACITY (instances above the capacity) or GSR_ERR_KERNEL (a bounded
+ * wait gave up); the failed forward's saved state is then dropped. */
 int         gsr_synchronize(gsr_context* ctx);
 /* The context's HIP stream (hipStream_t) so a caller can order its own work. */
 void*       gsr_stream(gsr_context* ctx);
@@ -117,8 +121,10 @@ int gsr_render(gsr_context* ctx, const gsr_camera* cam, const gsr_options* opts,
 /*
  * Forward from caller-owned planar device parameters ([59][N], see
  * gsr_scene_upload) — the entry the differentiable harness uses.  Enqueues
- * on the context stream and returns without waiting.  When
- * opts->save_state is 1 the per-pixel state for gsr_backward is kept.
+ * on the context stream and returns without waiting; an instance overflow or
+ * kernel failure of the frame is reported by the next gsr_synchronize (or
+ * gsr_last_instance_count).  When opts->save_state is 1 the per-pixel state
+ * for gsr_backward is kept.
  */
 int gsr_forward(gsr_context* ctx, const float* params_dev, uint32_t count,
                 const gsr_camera* cam, const gsr_options* opts, float* out_rgb_dev);
@@ -195,8 +201,10 @@ int gsr_rasterize(gsr_context* ctx, uint32_t width, uint32_t height, uint32_t ti
 
 /* ---- differentiation (no reference counterpart: the reference is forward-only) ---- */
 /*
- * Gradients of the last gsr_forward (which must have had save_state = 1,
- * and the same params/count/camera) with respect to the planar parameters.
+ * Gradients of the last gsr_forward (which must have had save_state = 1, and
+ * the same params pointer, count and camera, else GSR_ERR_STATE) with respect
+ * to the planar parameters.  A backward enqueued behind a forward that failed
+ * (see gsr_forward) writes quiet NaNs into grad_params_dev and grad2d_dev.
  * dL_drgb_dev: f32[3][H][W].  grad_params_dev: f32[59][N], overwritten.
  * grad2d_dev (may be NULL) receives the per-splat 2-D gradient, f32[N][12]:
  * pixel x, y; conic A, B, C; opacity; r, g, b; 3 unused.  Enqueues on the
@@ -229,6 +237,12 @@ int gsr_render_batch(gsr_context* ctx, const float* params_dev, uint32_t count,
 
 /* Tile instances of the last batch frame (raw scan total over all views). */
 int gsr_batch_instance_count(gsr_context* ctx, uint64_t* instances);
+
+/* Instance capacity (key/value slots of the sort buffers) the last batch frame
+ * ran with: opts->instance_capacity, else the planner's (16 per splat-view,
+ * raised to 1.25x the measured count after an overflow).  The reference sizes
+ * these buffers at 64 per splat (renderer.js:16, 536-577). */
+int gsr_batch_capacity(gsr_context* ctx, uint64_t* slots);
 
 /*
  * Autotuner for a batch shape: times tile size {8, 16, 32} x persistent raster

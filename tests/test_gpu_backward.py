@@ -144,6 +144,56 @@ def test_backward_requires_saved_state(torch_cuda, ctx):
     assert e.value.code == lib.GSR_ERR_STATE
 
 
+def test_backward_of_overflowed_forward_reports_capacity(torch_cuda, ctx):
+    """gsr_forward is asynchronous: an instance overflow of the forward is
+    reported by the next gsr_synchronize, a backward enqueued behind the failed
+    forward writes NaN gradients instead of replaying a truncated frame, and
+    the failed frame's state is dropped."""
+    sc, cam = scenes.workload_small()
+    u = cam.uniform(sc.count)
+    params = to_dev(torch_cuda, sc.planar())
+    out = torch_cuda.empty(3 * 512 * 512, dtype=torch_cuda.float32, device="cuda")
+    grad = torch_cuda.zeros_like(params)
+    g2 = torch_cuda.zeros(sc.count * 12, dtype=torch_cuda.float32, device="cuda")
+    d = torch_cuda.ones(3 * 512 * 512, dtype=torch_cuda.float32, device="cuda")
+    torch_cuda.cuda.synchronize()
+    ctx.forward(params, sc.count, u, lib.options(save_state=True, capacity=1000), out)
+    ctx.backward(params, sc.count, u, d, grad, g2)
+    with pytest.raises(lib.GsrError) as e:
+        ctx.synchronize()
+    assert e.value.code == lib.GSR_ERR_CAPACITY and "84916" in str(e.value)
+    assert torch_cuda.isnan(grad).all().item() and torch_cuda.isnan(g2).all().item()
+    with pytest.raises(lib.GsrError) as e:
+        ctx.backward(params, sc.count, u, d, grad)
+    assert e.value.code == lib.GSR_ERR_STATE
+    ctx.synchronize()   # nothing left to report
+    ctx.forward(params, sc.count, u, lib.options(save_state=True), out)
+    ctx.backward(params, sc.count, u, d, grad)
+    ctx.synchronize()
+    assert torch_cuda.isfinite(grad).all().item()
+
+
+def test_backward_rejects_another_camera_or_parameters(torch_cuda, ctx):
+    """The raster adjoint replays the saved forward while the preprocess adjoint
+    recomputes the projection from its arguments: a backward with a moved camera
+    of the same size, or with other parameters, is refused."""
+    sc, cam = scenes.workload_small()
+    u = cam.uniform(sc.count)
+    moved = scenes.Camera((0.1, 0.0, -4.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 60.0, 512, 512).uniform(sc.count)
+    params = to_dev(torch_cuda, sc.planar())
+    other = params.clone()
+    grad = torch_cuda.empty_like(params)
+    d = torch_cuda.zeros(3 * 512 * 512, dtype=torch_cuda.float32, device="cuda")
+    torch_cuda.cuda.synchronize()
+    ctx.forward(params, sc.count, u, lib.options(save_state=True), None)
+    for cam_u, p in ((moved, params), (u, other)):
+        with pytest.raises(lib.GsrError) as e:
+            ctx.backward(p, sc.count, cam_u, d, grad)
+        assert e.value.code == lib.GSR_ERR_STATE
+    ctx.backward(params, sc.count, u, d, grad)
+    ctx.synchronize()
+
+
 def test_backward_1m_1080p(torch_cuda, ctx, orc):
     sc, cam = scenes.workload_1m_1080p()
     u = cam.uniform(sc.count)

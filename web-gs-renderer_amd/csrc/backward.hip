@@ -17,6 +17,7 @@
 //
 // 2-D gradient layout per splat (12 floats): d/dx, d/dy of the pixel
 // position; d/dA, d/dB, d/dC of the conic; d/d opacity; d/d(r, g, b); pad.
+#include <cstddef>
 #include <cstring>
 
 #include "cull.h"
@@ -224,21 +225,28 @@ __device__ __forceinline__ void backward_block(uint32_t W, uint32_t H, uint32_t 
     }
 }
 
+// True when the forward whose state the backward replays overflowed its
+// instance capacity or had a bounded wait give up: its sorted instances,
+// ranges and per-pixel state are then incomplete.
+__device__ __forceinline__ bool frame_failed(const FrameCounters* counters, uint64_t capacity) {
+    return counters->instances > capacity || counters->error != 0u;
+}
+
 template <int TS>
 __global__ __launch_bounds__(256) void k_raster_backward(
-    uint32_t W, uint32_t H, uint32_t tiles_x, uint32_t num_tiles, uint32_t splats, const uint32_t* n_dev,
-    const uint2* __restrict__ ranges, const uint32_t* __restrict__ vals, const SplatRec* __restrict__ rec,
-    const float* __restrict__ tfinal, const uint32_t* __restrict__ last, const float* __restrict__ dLdC,
-    float* __restrict__ grad2d, const uint32_t* __restrict__ order) {
+    uint32_t W, uint32_t H, uint32_t tiles_x, uint32_t num_tiles, uint32_t splats, const FrameCounters* counters,
+    uint64_t capacity, const uint2* __restrict__ ranges, const uint32_t* __restrict__ vals,
+    const SplatRec* __restrict__ rec, const float* __restrict__ tfinal, const uint32_t* __restrict__ last,
+    const float* __restrict__ dLdC, float* __restrict__ grad2d, const uint32_t* __restrict__ order) {
     constexpr uint32_t SIDE = TS / 8, BLOCKS = SIDE * SIDE;
     const uint32_t wv = blockIdx.x * 4 + (threadIdx.x >> 6);
     const uint32_t slot = wv / BLOCKS, sub = wv % BLOCKS;
-    if (slot >= num_tiles) return;
+    if (slot >= num_tiles || frame_failed(counters, capacity)) return;
     const uint32_t tile = order[slot];
     const uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
     const uint2 r = ranges[tile];
-    backward_block(W, H, tx * TS + (sub % SIDE) * 8, ty * TS + (sub / SIDE) * 8, r.x, min(r.y, *n_dev), splats, vals,
-                   rec, tfinal, last, dLdC, grad2d);
+    backward_block(W, H, tx * TS + (sub % SIDE) * 8, ty * TS + (sub / SIDE) * 8, r.x, min(r.y, counters->n), splats,
+                   vals, rec, tfinal, last, dLdC, grad2d);
 }
 
 // View batches: the tiles of every view, view by view and heaviest first
@@ -534,13 +542,23 @@ __device__ __forceinline__ float opacity_of(const Par& par) {
 
 // grad2d arrives with the raster's per-splat sums and leaves in the 2-D
 // gradient layout (moments_to_2d); grad receives the 59 parameter gradients.
+// When the forward failed (frame_failed) every output is a quiet NaN, so a
+// caller that skips the error gsr_synchronize reports cannot take the result
+// for a gradient.
 __global__ __launch_bounds__(256) void k_preprocess_backward(const float* __restrict__ P, uint32_t n, CamK cam,
                                                              const SplatRec* __restrict__ rec,
-                                                             float4* __restrict__ grad2d,
+                                                             const FrameCounters* __restrict__ counters,
+                                                             uint64_t capacity, float4* __restrict__ grad2d,
                                                              float* __restrict__ grad) {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
     const size_t N = n;
+    if (frame_failed(counters, capacity)) {
+        const float q = __builtin_nanf("");
+        for (int k = 0; k < 3; ++k) grad2d[(size_t)g * 3 + k] = make_float4(q, q, q, q);
+        for (int ch = 0; ch < (int)kParams; ++ch) grad[(size_t)ch * N + g] = q;
+        return;
+    }
     // every load up front (sums, conic, the 59 parameters): one memory round
     // trip per thread instead of one per dependent stage
     const float4 G0 = grad2d[(size_t)g * 3], G1 = grad2d[(size_t)g * 3 + 1], G2 = grad2d[(size_t)g * 3 + 2];
@@ -716,15 +734,15 @@ int launch_backward(Context* c, const float* params, uint32_t count, const CamK&
     switch (tile) {
         case 8:
             hipLaunchKernelGGL(k_raster_backward<8>, dim3(blocks), dim3(256), 0, c->stream, W, H, tx, tiles, count,
-                               &counters->n, ranges, vals, rec, tfinal, last, dLdC, g2, order);
+                               counters, c->state_capacity, ranges, vals, rec, tfinal, last, dLdC, g2, order);
             break;
         case 16:
             hipLaunchKernelGGL(k_raster_backward<16>, dim3(blocks), dim3(256), 0, c->stream, W, H, tx, tiles, count,
-                               &counters->n, ranges, vals, rec, tfinal, last, dLdC, g2, order);
+                               counters, c->state_capacity, ranges, vals, rec, tfinal, last, dLdC, g2, order);
             break;
         case 32:
             hipLaunchKernelGGL(k_raster_backward<32>, dim3(blocks), dim3(256), 0, c->stream, W, H, tx, tiles, count,
-                               &counters->n, ranges, vals, rec, tfinal, last, dLdC, g2, order);
+                               counters, c->state_capacity, ranges, vals, rec, tfinal, last, dLdC, g2, order);
             break;
         default:
             return fail(c, GSR_ERR_STATE, "bad saved tile size %u", tile);
@@ -732,7 +750,7 @@ int launch_backward(Context* c, const float* params, uint32_t count, const CamK&
     GSR_HIP(c, hipGetLastError());
     mark(c, "backward preprocess");
     hipLaunchKernelGGL(k_preprocess_backward, dim3(ceil_div(count, 256)), dim3(256), 0, c->stream, params, count, cam,
-                       rec, (float4*)g2, grad);
+                       rec, counters, c->state_capacity, (float4*)g2, grad);
     GSR_HIP(c, hipGetLastError());
     mark_end(c, false);
     if (grad2d_out) GSR_HIP(c, hipMemcpyAsync(grad2d_out, g2, N * 48, hipMemcpyDeviceToDevice, c->stream));
@@ -803,10 +821,16 @@ int gsr_backward(gsr_context* ctx, const float* params_dev, uint32_t count, cons
     if (!c->have_state || c->state_count != count || c->state_w != (uint32_t)cam->texture_size[0] ||
         c->state_h != (uint32_t)cam->texture_size[1])
         return fail(c, GSR_ERR_STATE, "gsr_backward needs the matching gsr_forward with save_state = 1 first");
-    hipSetDevice(c->device);
     CamK ck;
     std::memcpy(&ck, cam, sizeof(ck));
     ck.count = count;
+    // the raster adjoint replays the saved forward and the preprocess adjoint
+    // recomputes the projection from these arguments: both must be the forward's
+    if (std::memcmp(&ck, &c->state_cam, offsetof(CamK, pad)) != 0)
+        return fail(c, GSR_ERR_STATE, "gsr_backward: the camera differs from the saved forward's");
+    if (count > 0 && params_dev != c->state_params)
+        return fail(c, GSR_ERR_STATE, "gsr_backward: the parameters are not the saved forward's");
+    hipSetDevice(c->device);
     return launch_backward(c, params_dev, count, ck, dL_drgb_dev, grad_params_dev, grad2d_dev);
 }
 

@@ -293,6 +293,7 @@ static int run_batch(Context* c, const float* params, uint32_t count, const gsr_
             return fail(c, GSR_ERR_KERNEL, "a look-back wait exceeded its bound (code %u)", c->host_counters->error);
         const uint64_t inst = c->host_counters->instances;
         c->batch_last_instances = inst;
+        c->batch_last_capacity = p.capacity;
         if (inst <= p.capacity) {
             if (use_graph && !replay) {
                 if ((rc = capture_batch(c, params, count, p, out))) return rc;
@@ -333,6 +334,14 @@ int gsr_batch_instance_count(gsr_context* ctx, uint64_t* instances) {
     if (!c) return fail(nullptr, GSR_ERR_ARGUMENT, "null context");
     if (!instances) return fail(c, GSR_ERR_ARGUMENT, "null output");
     *instances = c->batch_last_instances;
+    return GSR_OK;
+}
+
+int gsr_batch_capacity(gsr_context* ctx, uint64_t* slots) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    if (!c) return fail(nullptr, GSR_ERR_ARGUMENT, "null context");
+    if (!slots) return fail(c, GSR_ERR_ARGUMENT, "null output");
+    *slots = c->batch_last_capacity;
     return GSR_OK;
 }
 

@@ -203,6 +203,8 @@ const char* gsr_last_error(const gsr_context* ctx) {
 int gsr_synchronize(gsr_context* ctx) {
     Context* c = reinterpret_cast<Context*>(ctx);
     if (!c) return fail(nullptr, GSR_ERR_ARGUMENT, "null context");
+    hipSetDevice(c->device);
+    if (c->frame_unchecked) return check_frame(c, c->state_capacity);
     GSR_HIP(c, hipStreamSynchronize(c->stream));
     return GSR_OK;
 }

@@ -164,23 +164,34 @@ struct Context {
     std::vector<gsr_timing_row> last_rows;
     int last_frames = 0;
     uint32_t timing_on = 0;  // gsr_options.timing of the current call
-    // saved forward state for backward
+    // saved forward state for backward: the forward's camera (count filled
+    // in) and parameters, which gsr_backward must be called with
     bool have_state = false;
     uint32_t state_count = 0, state_w = 0, state_h = 0, state_tile = 16, state_key_bytes = 4;
     uint64_t state_capacity = 0;
     int state_sorted_index = 0;
+    CamK state_cam{};
+    const float* state_params = nullptr;
+    // a gsr_forward whose counters (instance overflow, look-back failure) no
+    // host call has read yet; gsr_synchronize reads them
+    bool frame_unchecked = false;
     uint32_t last_frame_w = 0, last_frame_h = 0;  // size of the frame in "image"
     // prefix of the scan/sort scratch buffer names ("" plain frames, "b_" batches)
     const char* scratch_prefix = "";
     // batches
     void* staging = nullptr;   // pinned per-view tables
     size_t staging_bytes = 0;
-    uint64_t batch_capacity_hint = 0, batch_last_instances = 0;
+    uint64_t batch_capacity_hint = 0, batch_last_instances = 0, batch_last_capacity = 0;
     BatchGraph graph;
     BatchState bstate;
     // host pinned staging for small readbacks
     FrameCounters* host_counters = nullptr;
 };
+
+// Reads the frame counters back (stream synced) and reports overflow of
+// `capacity` (GSR_ERR_CAPACITY) or a failed bounded wait (GSR_ERR_KERNEL); a
+// failed frame's saved state is dropped.
+int check_frame(Context* c, uint64_t capacity);
 
 // Named device buffer, grown to at least `bytes` (contents not preserved).
 // Growing a buffer drops any captured batch graph (it may point at it).

@@ -94,6 +94,9 @@ int enqueue_forward(Context* c, const float* params, uint32_t count, const gsr_c
             c->last_frame_h = p.H;
         }
         c->have_state = save_state;
+        c->state_params = params;
+        std::memcpy(&c->state_cam, cam, sizeof(CamK));
+        c->state_cam.count = 0;
         c->state_count = 0;
         c->state_w = p.W;
         c->state_h = p.H;
@@ -144,6 +147,8 @@ int enqueue_forward(Context* c, const float* params, uint32_t count, const gsr_c
         c->last_frame_h = p.H;
     }
     c->have_state = save_state;
+    c->state_params = params;
+    c->state_cam = ck;
     c->state_count = count;
     c->state_w = p.W;
     c->state_h = p.H;
@@ -154,18 +159,22 @@ int enqueue_forward(Context* c, const float* params, uint32_t count, const gsr_c
     return GSR_OK;
 }
 
-// Reads the frame counters back (stream synced) and reports overflow/errors.
 int check_frame(Context* c, uint64_t capacity) {
     FrameCounters* counters = (FrameCounters*)buffer(c, "counters", sizeof(FrameCounters));
     if (!counters) return GSR_ERR_ALLOC;
+    c->frame_unchecked = false;
     GSR_HIP(c, hipMemcpyAsync(c->host_counters, counters, sizeof(FrameCounters), hipMemcpyDeviceToHost, c->stream));
     GSR_HIP(c, hipStreamSynchronize(c->stream));
     finish_frame_timings(c);
-    if (c->host_counters->error)
+    if (c->host_counters->error) {
+        c->have_state = false;
         return fail(c, GSR_ERR_KERNEL, "a look-back wait exceeded its bound (code %u)", c->host_counters->error);
-    if (c->host_counters->instances > capacity)
+    }
+    if (c->host_counters->instances > capacity) {
+        c->have_state = false;
         return fail(c, GSR_ERR_CAPACITY, "%llu tile instances exceed the instance capacity %llu",
                     (unsigned long long)c->host_counters->instances, (unsigned long long)capacity);
+    }
     return GSR_OK;
 }
 
@@ -191,8 +200,12 @@ int gsr_forward(gsr_context* ctx, const float* params_dev, uint32_t count, const
         GSR_HIP(c, hipStreamSynchronize(c->stream));
         finish_frame_timings(c);
     }
+    c->have_state = false;
     rc = enqueue_forward(c, params_dev, count, cam, p, opts && opts->save_state, nullptr);
     if (rc) return rc;
+    // counters are read by the next gsr_synchronize (or instance-count query);
+    // a backward enqueued before that checks them on the device
+    c->frame_unchecked = true;
     if (out_rgb_dev) {
         float* image = (float*)buffer(c, "image", (size_t)p.W * p.H * 12);
         GSR_HIP(c, hipMemcpyAsync(out_rgb_dev, image, (size_t)p.W * p.H * 12, hipMemcpyDeviceToDevice, c->stream));
@@ -220,6 +233,7 @@ int gsr_render(gsr_context* ctx, const gsr_camera* cam, const gsr_options* opts,
     int rc = plan_frame(c, it->second.count, cam, opts, &p);
     if (rc) return rc;
     c->timing_on = opts ? opts->timing : 0;
+    c->have_state = false;
     rc = enqueue_forward(c, it->second.params, it->second.count, cam, p, opts && opts->save_state, nullptr);
     if (rc) return rc;
     const size_t bytes = (size_t)p.W * p.H * 12;

@@ -108,6 +108,7 @@ def load():
     L.gsr_render_batch.argtypes = [_VP, _VP, _U32, ctypes.POINTER(Camera), _U32, ctypes.POINTER(Options), _INT,
                                    _VP, ctypes.POINTER(_U64)]
     L.gsr_batch_instance_count.argtypes = [_VP, ctypes.POINTER(_U64)]
+    L.gsr_batch_capacity.argtypes = [_VP, ctypes.POINTER(_U64)]
     L.gsr_backward_batch.argtypes = [_VP, _VP, _U32, ctypes.POINTER(Camera), _U32, _VP, _VP, _VP]
     L.gsr_autotune_batch.argtypes = [_VP, _VP, _U32, ctypes.POINTER(Camera), _U32, ctypes.POINTER(Options), _VP,
                                      ctypes.POINTER(ctypes.c_double)]
@@ -239,6 +240,11 @@ class Context:
     def batch_instance_count(self) -> int:
         v = _U64()
         self._check(self.L.gsr_batch_instance_count(self.h, ctypes.byref(v)))
+        return v.value
+
+    def batch_capacity(self) -> int:
+        v = _U64()
+        self._check(self.L.gsr_batch_capacity(self.h, ctypes.byref(v)))
         return v.value
 
     def autotune_batch(self, params_dev, count: int, uniforms: list, opts: Options, out_dev):
