@@ -183,6 +183,20 @@ int gsr_sort_pairs_geometry(gsr_context* ctx, uint32_t key_bytes, void* keys0_de
                             uint32_t begin_bit, uint32_t end_bit, uint32_t blocks_per_cu,
                             uint32_t digit_bits, int* result_index);
 
+/* gsr_sort_pairs (1 block per CU, 8-bit digits, the same result) with no
+ * device allocation: no reference counterpart; it is for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Its scratch (counters, digit histograms, look-back
+ * words) is one fixed array of GSR_SORT_NOALLOC_SCRATCH_BYTES per GPU, held
+ * until the process ends, which bounds n: at most 133,955,584 keys of 4 bytes
+ * or 92,094,464 of 8 bytes (larger n: GSR_ERR_ARGUMENT, nothing launched).
+ * Every call on a GPU shares that array, so calls that overlap on one GPU
+ * (from several contexts or threads) race; calls in sequence are safe. */
+#define GSR_SORT_NOALLOC_SCRATCH_BYTES (16u << 20)
+int gsr_sort_pairs_noalloc(gsr_context* ctx, uint32_t key_bytes, void* keys0_dev, uint32_t* values0_dev,
+                           void* keys1_dev, uint32_t* values1_dev, uint64_t n,
+                           uint32_t begin_bit, uint32_t end_bit, int* result_index);
+
 /* tile_ranges.wgsl:7-35: ranges[2t] = first, ranges[2t+1] = last+1 of tile t
  * (= key >> 16) over sorted keys[0..n); untouched tiles are (0,0). */
 int gsr_tile_ranges(gsr_context* ctx, const void* keys_dev, uint32_t key_bytes, uint64_t n,

@@ -171,6 +171,7 @@ int gsr_create(gsr_context** out, int device_ordinal) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device_ordinal) == hipSuccess && prop.multiProcessorCount > 0)
         c->cu_count = prop.multiProcessorCount;
+    noalloc_attach(c);
     if (hipHostMalloc((void**)&c->host_counters, sizeof(FrameCounters), hipHostMallocDefault) != hipSuccess) {
         hipStreamDestroy(c->stream);
         delete c;

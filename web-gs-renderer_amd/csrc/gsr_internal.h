@@ -186,7 +186,15 @@ struct Context {
     BatchState bstate;
     // host pinned staging for small readbacks
     FrameCounters* host_counters = nullptr;
+    // device address of this device's fixed no-allocation scratch (null when
+    // its symbol lookup failed; noalloc_message says why)
+    uint8_t* noalloc_scratch = nullptr;
+    std::string noalloc_message;
 };
+
+// Looks up (once per device) the device address of the fixed scratch array
+// that gsr_sort_pairs_noalloc uses, with c's device current.
+void noalloc_attach(Context* c);
 
 // Reads the frame counters back (stream synced) and reports overflow of
 // `capacity` (GSR_ERR_CAPACITY) or a failed bounded wait (GSR_ERR_KERNEL); a
@@ -251,10 +259,28 @@ int launch_emit(Context* c, const uint32_t* offsets, const uint4* meta, uint32_t
 // result index (which ping-pong buffer holds the output).  hist_cleared: an
 // earlier kernel of the stream zeroed sort_hist_region (for the same bit
 // range and digit width), so the sort skips its own clearing launch.
+// mem (nullable): caller-provided scratch for the histograms and look-back
+// words instead of the context's buffers (no allocation); it must hold
+// sort_scratch_words() of each kind.
+struct SortMem {
+    uint32_t* hist;
+    size_t hist_words;
+    uint32_t* status;
+    size_t status_words;
+};
+struct SortWords {
+    size_t hist, status;
+};
 int launch_sort(Context* c, uint32_t key_bytes, void* k0, uint32_t* v0, void* k1, uint32_t* v1,
                 const uint32_t* n_dev, uint64_t n_host, uint64_t capacity, uint32_t begin_bit,
                 uint32_t end_bit, uint32_t blocks_per_cu, FrameCounters* counters, int* result_index,
-                uint32_t digit_bits = 8, bool hist_cleared = false);
+                uint32_t digit_bits = 8, bool hist_cleared = false, const SortMem* mem = nullptr);
+// Scratch words launch_sort needs for `capacity` keys with 1 block per CU and
+// 8-bit digits (other geometries: SIZE_MAX).
+SortWords sort_scratch_words(const Context* c, uint32_t key_bytes, uint64_t capacity, uint32_t begin_bit,
+                             uint32_t end_bit, uint32_t blocks_per_cu, uint32_t digit_bits);
+// Keys per full onesweep tile of that geometry.
+uint32_t sort_default_tile_keys(uint32_t key_bytes);
 // The sort's histogram and ticket words for this bit range and digit width
 // (count 0 when the buffer cannot be allocated).
 FillRegion sort_hist_region(Context* c, uint32_t begin_bit, uint32_t end_bit, uint32_t digit_bits);

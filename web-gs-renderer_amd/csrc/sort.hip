@@ -502,25 +502,45 @@ __global__ __launch_bounds__(THREADS) void k_onesweep(
         onesweep_tiles<K, THREADS, RB, I4>(L, kin, vin, kout, vout, n, num_tiles, shift, mask, ticket, status, error);
 }
 
+// Words of sort scratch for a geometry: the histograms and tickets, and two
+// sets of look-back words.  A pass has at most max(resident blocks, full
+// tiles of the capacity) tiles: the tile is shortened to give each resident
+// block one tile while the keys do not fill them all (onesweep_items).
+template <typename K, int THREADS, int RB>
+SortWords sort_words_of(const Context* c, uint64_t capacity, uint32_t begin_bit, uint32_t end_bit) {
+    using G = SortGeom<K, THREADS, RB>;
+    const uint32_t slots = (uint32_t)c->cu_count * (1024 / THREADS);
+    const uint64_t max_tiles = std::max<uint64_t>(slots, (capacity + G::kTile - 1) / G::kTile);
+    return SortWords{(size_t)sort_passes(begin_bit, end_bit, RB) * G::kRadixN + kMaxPasses,
+                     (size_t)2 * max_tiles * G::kRadixN};
+}
+
 template <typename K, int THREADS, int RB>
 int sort_impl(Context* c, K* k0, uint32_t* v0, K* k1, uint32_t* v1, const uint32_t* n_dev, uint64_t n_host,
               uint64_t capacity, uint32_t begin_bit, uint32_t end_bit, FrameCounters* counters,
-              int* result_index, bool hist_cleared) {
+              int* result_index, bool hist_cleared, const SortMem* mem) {
     using G = SortGeom<K, THREADS, RB>;
     constexpr uint32_t R = G::kRadixN;
     const uint32_t passes = sort_passes(begin_bit, end_bit, RB);
     *result_index = (int)(passes & 1u);
     if (passes == 0) return GSR_OK;
     if (passes > (uint32_t)kMaxPasses) return fail(c, GSR_ERR_ARGUMENT, "too many radix passes (%u)", passes);
-    // resident onesweep blocks; the tile is shortened to give each one tile
-    // while the keys do not fill them all (onesweep_items), so a pass has at
-    // most max(slots, full tiles of the capacity) tiles
     const uint32_t slots = (uint32_t)c->cu_count * (1024 / THREADS);
-    const uint32_t max_tiles = std::max<uint32_t>(slots, ceil_div(capacity, (uint64_t)G::kTile));
-    const size_t hist_words = (size_t)passes * R + kMaxPasses;
-    uint32_t* hist = (uint32_t*)scratch(c, "sort_hist", hist_words * 4);
-    // two sets of look-back words, alternating between passes
-    uint32_t* status = (uint32_t*)scratch(c, "sort_status", (size_t)2 * max_tiles * R * 4);
+    const SortWords words = sort_words_of<K, THREADS, RB>(c, capacity, begin_bit, end_bit);
+    const size_t hist_words = words.hist;
+    const uint32_t max_tiles = (uint32_t)(words.status / (2 * R));
+    uint32_t* hist;
+    uint32_t* status;   // two sets of look-back words, alternating between passes
+    if (mem) {
+        if (mem->hist_words < words.hist || mem->status_words < words.status)
+            return fail(c, GSR_ERR_ARGUMENT, "sort scratch too small: %zu + %zu words needed, %zu + %zu given",
+                        words.hist, words.status, mem->hist_words, mem->status_words);
+        hist = mem->hist;
+        status = mem->status;
+    } else {
+        hist = (uint32_t*)scratch(c, "sort_hist", hist_words * 4);
+        status = (uint32_t*)scratch(c, "sort_status", words.status * 4);
+    }
     if (!hist || !status) return GSR_ERR_ALLOC;
     uint32_t* tickets = hist + (size_t)passes * R;
     if (!hist_cleared)
@@ -552,17 +572,17 @@ int sort_impl(Context* c, K* k0, uint32_t* v0, K* k1, uint32_t* v1, const uint32
 template <typename K, int RB>
 int sort_geometry(Context* c, K* k0, uint32_t* v0, K* k1, uint32_t* v1, const uint32_t* n_dev, uint64_t n_host,
                   uint64_t capacity, uint32_t begin_bit, uint32_t end_bit, uint32_t blocks_per_cu,
-                  FrameCounters* counters, int* result_index, bool hist_cleared) {
+                  FrameCounters* counters, int* result_index, bool hist_cleared, const SortMem* mem) {
     switch (blocks_per_cu) {
         case 1:
             return sort_impl<K, 1024, RB>(c, k0, v0, k1, v1, n_dev, n_host, capacity, begin_bit, end_bit, counters,
-                                          result_index, hist_cleared);
+                                          result_index, hist_cleared, mem);
         case 2:
             return sort_impl<K, 512, RB>(c, k0, v0, k1, v1, n_dev, n_host, capacity, begin_bit, end_bit, counters,
-                                          result_index, hist_cleared);
+                                          result_index, hist_cleared, mem);
         case 4:
             return sort_impl<K, 256, RB>(c, k0, v0, k1, v1, n_dev, n_host, capacity, begin_bit, end_bit, counters,
-                                          result_index, hist_cleared);
+                                          result_index, hist_cleared, mem);
         default:
             return fail(c, GSR_ERR_ARGUMENT, "sort blocks per CU %u not in {1, 2, 4}", blocks_per_cu);
     }
@@ -571,20 +591,20 @@ int sort_geometry(Context* c, K* k0, uint32_t* v0, K* k1, uint32_t* v1, const ui
 template <typename K>
 int sort_digits(Context* c, K* k0, uint32_t* v0, K* k1, uint32_t* v1, const uint32_t* n_dev, uint64_t n_host,
                 uint64_t capacity, uint32_t begin_bit, uint32_t end_bit, uint32_t blocks_per_cu, uint32_t digit_bits,
-                FrameCounters* counters, int* result_index, bool hist_cleared) {
+                FrameCounters* counters, int* result_index, bool hist_cleared, const SortMem* mem) {
     switch (digit_bits) {
         case 8:
             return sort_geometry<K, 8>(c, k0, v0, k1, v1, n_dev, n_host, capacity, begin_bit, end_bit, blocks_per_cu,
-                                       counters, result_index, hist_cleared);
+                                       counters, result_index, hist_cleared, mem);
         case 9:
             return sort_geometry<K, 9>(c, k0, v0, k1, v1, n_dev, n_host, capacity, begin_bit, end_bit, blocks_per_cu,
-                                       counters, result_index, hist_cleared);
+                                       counters, result_index, hist_cleared, mem);
         case 10:
             return sort_geometry<K, 10>(c, k0, v0, k1, v1, n_dev, n_host, capacity, begin_bit, end_bit,
-                                        blocks_per_cu, counters, result_index, hist_cleared);
+                                        blocks_per_cu, counters, result_index, hist_cleared, mem);
         case 11:
             return sort_geometry<K, 11>(c, k0, v0, k1, v1, n_dev, n_host, capacity, begin_bit, end_bit,
-                                        blocks_per_cu, counters, result_index, hist_cleared);
+                                        blocks_per_cu, counters, result_index, hist_cleared, mem);
         default:
             return fail(c, GSR_ERR_ARGUMENT, "sort digit width %u not in 8..11", digit_bits);
     }
@@ -612,19 +632,31 @@ FillRegion sort_hist_region(Context* c, uint32_t begin_bit, uint32_t end_bit, ui
     return FillRegion{p, p ? words : 0};
 }
 
+uint32_t sort_default_tile_keys(uint32_t key_bytes) {
+    return key_bytes == 8 ? (uint32_t)SortGeom<uint64_t, 1024, 8>::kTile : (uint32_t)SortGeom<uint32_t, 1024, 8>::kTile;
+}
+
+SortWords sort_scratch_words(const Context* c, uint32_t key_bytes, uint64_t capacity, uint32_t begin_bit,
+                             uint32_t end_bit, uint32_t blocks_per_cu, uint32_t digit_bits) {
+    if (digit_bits == 8 && blocks_per_cu == 1)
+        return key_bytes == 8 ? sort_words_of<uint64_t, 1024, 8>(c, capacity, begin_bit, end_bit)
+                              : sort_words_of<uint32_t, 1024, 8>(c, capacity, begin_bit, end_bit);
+    return SortWords{~(size_t)0, ~(size_t)0};  // only the default geometry is sized here
+}
+
 int launch_sort(Context* c, uint32_t key_bytes, void* k0, uint32_t* v0, void* k1, uint32_t* v1,
                 const uint32_t* n_dev, uint64_t n_host, uint64_t capacity, uint32_t begin_bit,
                 uint32_t end_bit, uint32_t blocks_per_cu, FrameCounters* counters, int* result_index,
-                uint32_t digit_bits, bool hist_cleared) {
+                uint32_t digit_bits, bool hist_cleared, const SortMem* mem) {
     if (capacity > kMaxSortKeys) return fail(c, GSR_ERR_ARGUMENT, "sort capacity %llu exceeds %llu",
                                              (unsigned long long)capacity, (unsigned long long)kMaxSortKeys);
     if (end_bit > key_bytes * 8 || begin_bit > end_bit)
         return fail(c, GSR_ERR_ARGUMENT, "bad bit range [%u, %u) for %u-byte keys", begin_bit, end_bit, key_bytes);
     if (key_bytes == 8)
         return sort_digits<uint64_t>(c, (uint64_t*)k0, v0, (uint64_t*)k1, v1, n_dev, n_host, capacity, begin_bit,
-                                     end_bit, blocks_per_cu, digit_bits, counters, result_index, hist_cleared);
+                                     end_bit, blocks_per_cu, digit_bits, counters, result_index, hist_cleared, mem);
     return sort_digits<uint32_t>(c, (uint32_t*)k0, v0, (uint32_t*)k1, v1, n_dev, n_host, capacity, begin_bit,
-                                 end_bit, blocks_per_cu, digit_bits, counters, result_index, hist_cleared);
+                                 end_bit, blocks_per_cu, digit_bits, counters, result_index, hist_cleared, mem);
 }
 
 }  // namespace gsr

@@ -102,6 +102,7 @@ def load():
     L.gsr_sort_pairs.argtypes = [_VP, _U32, _VP, _VP, _VP, _VP, _U64, _U32, _U32, ctypes.POINTER(_INT)]
     L.gsr_sort_pairs_geometry.argtypes = [_VP, _U32, _VP, _VP, _VP, _VP, _U64, _U32, _U32, _U32, _U32,
                                           ctypes.POINTER(_INT)]
+    L.gsr_sort_pairs_noalloc.argtypes = [_VP, _U32, _VP, _VP, _VP, _VP, _U64, _U32, _U32, ctypes.POINTER(_INT)]
     L.gsr_tile_ranges.argtypes = [_VP, _VP, _U32, _U64, _U32, _VP]
     L.gsr_rasterize.argtypes = [_VP, _U32, _U32, _U32, _U32, _U64, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP]
     L.gsr_backward.argtypes = [_VP, _VP, _U32, ctypes.POINTER(Camera), _VP, _VP, _VP]
@@ -294,6 +295,13 @@ class Context:
             self._check(self.L.gsr_sort_pairs_geometry(self.h, key_bytes, _ptr(k0), _ptr(v0), _ptr(k1), _ptr(v1), n,
                                                        begin_bit, end_bit, blocks_per_cu or 1, digit_bits or 8,
                                                        ctypes.byref(idx)))
+        return idx.value
+
+    def sort_pairs_noalloc(self, key_bytes, k0, v0, k1, v1, n, begin_bit, end_bit) -> int:
+        """gsr_sort_pairs without device allocation (fixed per-GPU scratch)."""
+        idx = _INT()
+        self._check(self.L.gsr_sort_pairs_noalloc(self.h, key_bytes, _ptr(k0), _ptr(v0), _ptr(k1), _ptr(v1), n,
+                                                  begin_bit, end_bit, ctypes.byref(idx)))
         return idx.value
 
     def tile_ranges(self, keys, key_bytes, n, num_tiles, ranges):
