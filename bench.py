@@ -8,7 +8,10 @@ vertical FOV, 1920x1080, 16x16 tiles, seed 1.  Synthetic data.
 Prints one JSON line.  `roofline` describes the raster, the stage with the
 largest share of the frame (reported from a per-stage run), timed with HIP
 events around it on the renderer's stream over the timed region (events
-around every stage would add gaps of their own to the frames timed);
+around every stage would add gaps of their own to the frames timed).  It is
+VALU-bound: achieved = pixel-splat evaluations (the oracle's count in the CPU
+leg) x 15 FP32 operations / raster time, against 157.3 TFLOP/s; its HBM view
+(40 B per instance + 8 B per tile + 12 B per pixel) is the `hbm` member;
 `stage_ms` comes from a separate run with events around every stage.
 `cpu_baseline` is the oracle's FP32 CPU pipeline (a port of the reference
 WGSL pass graph, OpenMP) timed on this host on a bounded sample.
@@ -26,10 +29,16 @@ sys.path.insert(0, os.path.join(ROOT, "web-gs-renderer_amd"))
 
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
-# HBM bytes per launch of the forward's kernels at this workload, from the
+# FP32 operations per pixel-splat evaluation of the raster's loop (power,
+# alpha, the skip and stop tests, the transmittance and colour update: about
+# 14) plus the exponential, counted as one.
+RASTER_OPS_PER_EVALUATION = 15
+# HBM bytes per launch of the forward raster at this workload, from the
 # rocprofv3 FETCH_SIZE / WRITE_SIZE runs summarised in
-# profiles/forward_1m_1080p_pmc_traffic.md (calibrated against a plain copy).
-MEASURED_TRAFFIC = {"raster": 622588548, "preprocess": 304205540, "sort": 241460827}
+# profiles/forward_1m_1080p_pmc_traffic.md (calibrated against a plain copy);
+# a counter run cannot be part of this process, so the figure is the profile's.
+RASTER_TRAFFIC_BYTES = 622588548
+RASTER_TRAFFIC_SOURCE = "profiles/forward_1m_1080p_pmc_traffic.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)"
 
 
 def log(msg):
@@ -54,15 +63,34 @@ def algorithmic_bytes(stage: str, n_splats: int, instances: int, visible: int, w
         # boundaries: read the keys, write the ranges; heaviest-first order: read ranges twice, write the order
         return instances * key_bytes + tiles * 8.0 + tiles * 20.0
     if stage == "raster":
-        # per instance the value and the 48 B splat record; per tile its range and order slot; RGB out
-        return instances * 52.0 + tiles * 12.0 + width * height * 12.0
+        # per instance the value and the splat's conic + opacity 16, pixel 8, colour 12;
+        # per tile its range; RGB out
+        return instances * 40.0 + tiles * 8.0 + width * height * 12.0
     return 0.0
+
+
+def cpu_description():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return model, os.cpu_count() or 1, usable
 
 
 def cpu_baseline(sc, u, budget_s: float = 15.0):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    threads = min(os.cpu_count() or 1, 16)
+    model, nproc, usable = cpu_description()
+    threads = min(usable, 16)
     oracle.lib().orc_set_threads(threads)
     frames = 0
     t0 = time.perf_counter()
@@ -81,6 +109,7 @@ def cpu_baseline(sc, u, budget_s: float = 15.0):
     single = 1.0 / (time.perf_counter() - t1)
     oracle.lib().orc_set_threads(threads)
     return {"value": frames / dt, "unit": "frames/s", "cores": threads, "kind": "port",
+            "cpu_model": model, "nproc": nproc, "usable_cpus": usable,
             "single_thread_value": round(single, 4),
             "sample": f"{frames} full frames of the same 1M-splat 1080p workload through the oracle's FP32 "
                       f"C++ pass graph (OpenMP, {threads} threads, stable LSD radix sort), plus one frame on "
@@ -189,6 +218,9 @@ def main():
                     help="time only forward+backward steps (for short profiler runs); prints a short JSON line")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB five times (HBM counter calibration)")
+    ap.add_argument("--valu-probe", type=int, default=0, metavar="ITERS",
+                    help="only run the four VALU calibration kernels (ITERS loop iterations, 2048 workgroups of "
+                         "4 waves, a warm-up then one timed launch each) for VALU counter calibration")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the renderer has no "
@@ -209,6 +241,24 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes": src.numel() * 4, "copies": 5}))
+        return
+    if args.valu_probe:
+        import ctypes
+        calib = ctypes.CDLL(os.path.join(ROOT, "web-gs-renderer_amd", "build", "libgsr_calib.so"))
+        calib.gsr_calib_valu.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+        blocks = 2048
+        waves = blocks * 4
+        # VALU instructions per wave of each mode: per iteration, and around the
+        # loop (from the kernels' gfx950 assembly)
+        per_mode = {0: (8, 24), 1: (16, 43), 2: (16, 39), 3: (24, 43)}
+        res = {}
+        for mode, (body, fixed) in per_mode.items():
+            ms = ctypes.c_double()
+            rc = calib.gsr_calib_valu(mode, blocks, args.valu_probe, ctypes.byref(ms))
+            valu = waves * (body * args.valu_probe + fixed)
+            res[mode] = {"rc": rc, "valu_wave_instructions": valu, "ms": round(ms.value, 5),
+                         "wave_instructions_per_ns": round(valu / (ms.value * 1e6), 3)}
+        print(json.dumps({"valu_probe": res, "waves_per_launch": waves, "iters": args.valu_probe}))
         return
     if args.scale_only:
         ctx = lib.Context(0)
@@ -302,11 +352,10 @@ def main():
     dominant = "raster"
     dom_ms = raster_ms
     dom_bytes = algorithmic_bytes(dominant, n, instances, visible, W, H, tiles, key_bytes, passes)
-    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": MEASURED_TRAFFIC.get(dominant),
-                "kernel": dominant, "kernel_ms": round(dom_ms, 5), "algorithmic_bytes": int(dom_bytes),
-                "largest_stage": largest}
+    hbm_gbs = dom_bytes / (dom_ms * 1e-3) / 1e9
+    hbm = {"algorithmic_bytes": int(dom_bytes), "achieved": round(hbm_gbs, 2), "peak": HBM_PEAK_GBS,
+           "unit": "GB/s", "frac": round(hbm_gbs / HBM_PEAK_GBS, 5),
+           "bytes_rule": "40 B per instance + 8 B per tile + 12 B per pixel"}
     sort_ms = stages.get("sort", 0.0)
     extra = {
         "stage_ms": {k: round(v, 5) for k, v in stage_ms.items()},
@@ -354,15 +403,26 @@ def main():
     if not args.no_scale:
         extra["scale_6m_8x4k"] = scale_leg(ctx, torch, lib, scenes)
     cpu = None
+    evals = None
     if not args.no_cpu:
         log("cpu baseline")
         cpu, evals = cpu_baseline(sc, u)
-        # the raster is VALU-bound: its algorithmic work is the pixel-splat
-        # evaluations under the early-termination rules (counted by the oracle)
-        rms = stages.get("raster", 0.0)
-        if rms:
-            extra["raster_valu"] = {"evaluations": evals, "evaluations_per_s": round(evals / (rms * 1e-3), 1),
-                                    "note": "pixel-splat evaluations per raster launch / raster time"}
+    # The raster is VALU-bound: its algorithmic work is the pixel-splat
+    # evaluations under the reference's early-termination rules (counted by
+    # the oracle's frame in the CPU leg) times the FP32 operations of one
+    # evaluation, against the FP32 peak; the HBM view rides along.
+    if evals:
+        tflops = evals * RASTER_OPS_PER_EVALUATION / (dom_ms * 1e-3) / 1e12
+        roofline = {"bound": "valu", "achieved": round(tflops, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": round(tflops / FP32_PEAK_TFLOPS, 5), "traffic": RASTER_TRAFFIC_BYTES,
+                    "traffic_source": RASTER_TRAFFIC_SOURCE, "kernel": dominant, "kernel_ms": round(dom_ms, 5),
+                    "evaluations": evals, "ops_per_evaluation": RASTER_OPS_PER_EVALUATION,
+                    "evaluations_per_s": round(evals / (dom_ms * 1e-3), 1), "hbm": hbm, "largest_stage": largest}
+    else:  # no CPU leg, no evaluation count: the HBM view only
+        roofline = {"bound": "hbm", **{k: hbm[k] for k in ("achieved", "peak", "unit", "frac")},
+                    "traffic": RASTER_TRAFFIC_BYTES, "traffic_source": RASTER_TRAFFIC_SOURCE, "kernel": dominant,
+                    "kernel_ms": round(dom_ms, 5), "algorithmic_bytes": hbm["algorithmic_bytes"],
+                    "largest_stage": largest}
     ctx.close()
     out = {
         "metric": "fwd & fwd+bwd frames/s @1M splats 1080p; 64-view batch vs plain; sort Gkeys/s",

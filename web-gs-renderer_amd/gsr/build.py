@@ -60,6 +60,19 @@ def build_library(verbose: bool = False) -> str:
     return LIB
 
 
+CALIB_SRC = os.path.join(PKG_DIR, "calib", "valu_probe.hip")
+CALIB_LIB = os.path.join(BUILD, "libgsr_calib.so")
+
+
+def build_calibration() -> str:
+    """Counter-calibration kernel for bench.py --valu-probe (not part of the renderer)."""
+    os.makedirs(BUILD, exist_ok=True)
+    tmp = CALIB_LIB + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, "-shared", CALIB_SRC, "-o", tmp])
+    os.replace(tmp, CALIB_LIB)
+    return CALIB_LIB
+
+
 def build_addon() -> str | None:
     """C N-API addon (Node >= 12) linked against libgsr.so; skipped without Node headers."""
     src = os.path.join(NAPI, "gsr_napi.c")
@@ -77,6 +90,7 @@ def build_addon() -> str | None:
 def build(verbose: bool = False):
     lib = build_library(verbose)
     addon = build_addon()
+    build_calibration()
     return lib, addon
 
 
