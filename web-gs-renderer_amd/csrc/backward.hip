@@ -8,9 +8,11 @@
 //   transmittance; alphas are recomputed with the forward's exact FP32 steps
 //   (so the blend decisions are the forward's), T is recovered as
 //   T * rcp(1 - alpha), and the light arriving from behind each splat is
-//   accumulated on the way (as one dL/dRGB-weighted sum per pixel).  Per splat the 9 gradient terms are summed over the wave (8 by a
-//   swap-and-add transposition, 1 by DPP) and added to the splat's 2-D
-//   gradient with float atomics, one instruction for the 8.
+//   accumulated on the way (as one dL/dRGB-weighted sum per pixel).  Each
+//   lane stores its pixel's dL/dpower and blend weight per blended splat in
+//   the wave's LDS; every few splats the wave sums them per splat over pixel
+//   rows (moments of dL/dpower, opacity and colour terms) and adds the nine
+//   sums to the splat's 2-D gradient with float atomics.
 // k_preprocess_backward: one thread per splat, turns the raster's sums into
 //   the 2-D gradient (pixel position, conic, opacity, colour) and chains it
 //   back to the 59 parameters.
@@ -33,68 +35,9 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
     return v;
 }
 
-// Sums of 8 per-lane values over the wave by pairwise transposition.  A
-// permlane32 swap exchanges the upper half of one register with the lower
-// half of another, so the sum of the pair's two results holds, in one half
-// of the wave, half-sums of the first value and in the other half those of
-// the second: 4 swaps and 4 adds fold 8 values into 4.  permlane16 swaps fold
-// 4 into 2 the same way on 16-lane rows; a DPP rotate by 8 folds 2 into 1 on
-// 8-lane groups; three DPP steps (half-row mirror, quad swaps) finish each
-// group.  Lane 8i then holds one value's wave total.  Which value lands where
-// depends only on the swap network, so the same network run once on the
-// values' indices (fold_tags) names it.
-
-__device__ __forceinline__ float fold32(float d, float s) {
-    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(d), __float_as_uint(s), false, false);
-    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
-}
-__device__ __forceinline__ float fold16(float d, float s) {
-    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(d), __float_as_uint(s), false, false);
-    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
-}
-__device__ __forceinline__ uint32_t tag32(uint32_t d, uint32_t s) {
-    return __builtin_amdgcn_permlane32_swap(d, s, false, false)[0];
-}
-__device__ __forceinline__ uint32_t tag16(uint32_t d, uint32_t s) {
-    return __builtin_amdgcn_permlane16_swap(d, s, false, false)[0];
-}
 template <int CTRL>
 __device__ __forceinline__ float dpp_f32(float v) {
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
-}
-
-__device__ __forceinline__ uint32_t fold_tags() {
-    const uint32_t u0 = tag32(0, 1), u1 = tag32(2, 3), u2 = tag32(4, 5), u3 = tag32(6, 7);
-    const uint32_t w0 = tag16(u0, u1), w1 = tag16(u2, u3);
-    return (threadIdx.x & 8) ? w1 : w0;
-}
-
-template <int CTRL, int ROW_MASK>
-__device__ __forceinline__ float dpp_f32_rows(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xF, false));
-}
-
-// The 8-value fold of v (s8: lane 8i holds the total of v[fold_tags() of
-// lane 8i]) and the plain DPP wave sum of vb (sb, total in lane 63), their DPP
-// steps interleaved so that each fills the other's read-after-write wait
-// states.
-__device__ __forceinline__ void fold9(const float (&v)[8], float vb, float& s8, float& sb) {
-    const float u0 = fold32(v[0], v[1]), u1 = fold32(v[2], v[3]);
-    const float u2 = fold32(v[4], v[5]), u3 = fold32(v[6], v[7]);
-    const float w0 = fold16(u0, u1), w1 = fold16(u2, u3);
-    const bool hi = (threadIdx.x & 8) != 0;
-    float z = (hi ? w1 : w0) + dpp_f32<0x128>(hi ? w0 : w1);  // row_ror:8
-    float b = vb + dpp_f32<0x111>(vb);                          // row_shr:1
-    z += dpp_f32<0x141>(z);                                     // row_half_mirror
-    b += dpp_f32<0x112>(b);                                     // row_shr:2
-    z += dpp_f32<0xB1>(z);                                      // quad_perm [1,0,3,2]
-    b += dpp_f32<0x114>(b);                                     // row_shr:4
-    z += dpp_f32<0x4E>(z);                                      // quad_perm [2,3,0,1]
-    b += dpp_f32<0x118>(b);                                     // row_shr:8
-    b += dpp_f32_rows<0x142, 0xA>(b);                           // row_bcast:15
-    b += dpp_f32_rows<0x143, 0xC>(b);                           // row_bcast:31
-    s8 = z;
-    sb = b;
 }
 
 // One gathered splat as staged in LDS (colour w carries the splat index).
@@ -105,20 +48,109 @@ struct SplatB {
     float2 pad;
 };
 
-// One 8x8 block of one tile per wave64 (one pixel per lane); workgroups of 4
-// waves take the tiles heaviest first (order[], as in the forward).  Each
-// wave stages its batch of 64 splats in its own LDS slots and reads them back
-// with same-address loads.
+// Splats whose per-pixel terms a wave collects before it sums them per
+// splat (see backward_block).
+constexpr int kBwdChunk = 8;
+
+// LDS of one wave of backward_block (the workgroup holds 4; 8 KiB each, so
+// that five workgroups fit a CU).  pair[c] holds collected splat c's terms
+// with pixel p at p ^ 8c, so that the flush's reads of different splats fall
+// in different banks.
+struct BwdWaveLds {
+    SplatB spl[64];                 // the staged batch
+    float2 pair[kBwdChunk][64];     // per collected splat and pixel: (gpow, w)
+    float4 g[64];                   // per pixel: dL/d(r, g, b)
+};
+static_assert(sizeof(BwdWaveLds) == 8192, "backward LDS per wave");
+
+// Sums the collected splats' terms per splat and adds them to the splats'
+// 2-D gradient sums.  Lane l works on collected splat c = l / 8 and block
+// column s = l % 8 (pixels s, s + 8, .., s + 56: dx is constant per lane);
+// the 8 lanes of a splat then combine their partial sums with three DPP
+// steps and add the nine totals with two atomic instructions (lane s the
+// total s, the first lane the ninth): an instruction's lanes address one
+// splat's entry each, as the atomics' cost is per addressed cache line.
+// slots: 6 bits per collected splat, its slot in the staged batch.
+__device__ __forceinline__ void backward_flush(BwdWaveLds& L, uint32_t n, uint64_t slots, uint32_t x0,
+                                               uint32_t y0, uint32_t splats, float* __restrict__ grad2d) {
+    static_assert(kBwdChunk == 8, "one splat per 8 lanes");
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t c = lane >> 3, col = lane & 7;
+    // the collected terms were written by every lane of the wave
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    float sg = 0.f, sy = 0.f, syy = 0.f, sr = 0.f, sgg = 0.f, sb = 0.f, dx = 0.f;
+    float op = 1.f;
+    uint32_t gid = 0xFFFFFFFFu;
+    if (c < n) {
+        const SplatB& sp = L.spl[(uint32_t)(slots >> (6 * c)) & 63u];
+        const float ix = sp.p.x, iy = sp.p.y;
+        op = sp.co.w;
+        gid = __float_as_uint(sp.cl.w);
+        dx = ix - (float)(x0 + col);
+        const float fy0 = (float)y0;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const float2 pr = L.pair[c][col + 8 * (i ^ c)];  // pixel col + 8 i
+            const float4 gv = L.g[col + 8 * i];
+            const float dy = iy - (fy0 + (float)i);
+            const float gy = pr.x * dy;
+            sg += pr.x;
+            sy += gy;
+            syy = __builtin_fmaf(gy, dy, syy);
+            sr = __builtin_fmaf(pr.y, gv.x, sr);
+            sgg = __builtin_fmaf(pr.y, gv.y, sgg);
+            sb = __builtin_fmaf(pr.y, gv.z, sb);
+        }
+    }
+    // moments with the lane's constant dx: sum g dx = dx sum g, and so on
+    float v[9] = {dx * sg, sy, dx * dx * sg, dx * sy, syy, sg, sr, sgg, sb};
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        float t = v[k];
+        t += dpp_f32<0xB1>(t);   // quad_perm [1,0,3,2]
+        t += dpp_f32<0x4E>(t);   // quad_perm [2,3,0,1]
+        t += dpp_f32<0x141>(t);  // row_half_mirror: the 8-lane total in every lane
+        v[k] = t;
+    }
+    // opacity: sum of e gal over the pixels off the 0.99 clamp = sum gpow / op
+    v[5] = v[5] / op;
+    // lane col of a splat's 8 lanes adds total col (one atomic instruction
+    // whose lanes address 8 consecutive floats per splat), the first lane
+    // the ninth
+    const bool b0 = col & 1, b1 = col & 2, b2 = col & 4;
+    const float s01 = b0 ? v[1] : v[0], s23 = b0 ? v[3] : v[2], s45 = b0 ? v[5] : v[4], s67 = b0 ? v[7] : v[6];
+    const float s03 = b1 ? s23 : s01, s47 = b1 ? s67 : s45;
+    const float mine = b2 ? s47 : s03;
+    // the collected slots are reused by the next chunk
+    __builtin_amdgcn_wave_barrier();
+    if (c < n && gid < splats) {
+        float* dst = grad2d + (size_t)gid * 12;
+        unsafeAtomicAdd(dst + col, mine);
+        if (col == 0) unsafeAtomicAdd(dst + 8, v[8]);
+    }
+}
+
 // One 8x8 block (pixels x0.. x0 + 7, y0.. y0 + 7 of an image of W x H whose
 // planes start at tfinal / last / dLdC) replaying the tile's instances
-// [start, min(range_end, start + max last contributor)) back to front.
-// grad2d receives, per entry of vals, the moment sums of the kernel comment.
+// [start, min(range_end, start + max last contributor)) back to front, one
+// pixel per lane, the batch of 64 splats staged in the wave's LDS and read
+// back with same-address loads.  For every splat that some pixel blended,
+// each lane stores its pixel's gpow = dL/dpower and w = alpha T into the
+// wave's chunk buffer; every kBwdChunk such splats (and at the end of each
+// batch, whose staged records the flush reads) backward_flush turns
+// the buffer into per-splat sums: the moments of
+// gpow (g dx, g dy, g dx^2, g dx dy, g dy^2), the opacity term and the colour
+// terms, added to grad2d per entry of vals.  Summing per splat over block
+// columns replaces a 64-lane reduction per splat and block by a few adds per
+// pixel and three DPP steps per sum.
 __device__ __forceinline__ void backward_block(uint32_t W, uint32_t H, uint32_t x0, uint32_t y0, uint32_t start,
                                                uint32_t range_end, uint32_t splats, const uint32_t* __restrict__ vals,
                                                const SplatRec* __restrict__ rec, const float* __restrict__ tfinal,
                                                const uint32_t* __restrict__ last, const float* __restrict__ dLdC,
                                                float* __restrict__ grad2d) {
-    __shared__ SplatB s_spl[4][64];
+    __shared__ BwdWaveLds s_bwd[4];
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t x = x0 + lane % 8, y = y0 + lane / 8;
     const float fx = (float)x, fy = (float)y;
@@ -141,12 +173,15 @@ __device__ __forceinline__ void backward_block(uint32_t W, uint32_t H, uint32_t 
     if (maxlast == 0) return;
     const uint32_t stop = min(range_end, start + maxlast);  // exclusive end of the replay
     if (stop <= start) return;
-    const uint32_t tag = fold_tags();
     const float rxa = (float)x0, rxb = (float)min(x0 + 7, W - 1);
     const float rya = (float)y0, ryb = (float)min(y0 + 7, H - 1);
-    // the wave's slot row as a wave-uniform (scalar) index, so that a
-    // record's LDS address is scalar arithmetic
+    // the wave's LDS as a wave-uniform (scalar) index, so that a record's
+    // LDS address is scalar arithmetic
     const uint32_t wv = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    BwdWaveLds& L = s_bwd[wv];
+    L.g[lane] = make_float4(gr, gg, gb, 0.0f);
+    uint32_t n = 0;       // splats collected in the chunk buffer (wave-uniform)
+    uint64_t slots = 0;   // their slots in the staged batch, 6 bits each
     for (int64_t b = (int64_t)start + (int64_t)((stop - start - 1) / 64) * 64; b >= (int64_t)start; b -= 64) {
         const uint32_t idx = (uint32_t)b + lane;
         float4 co = make_float4(0.f, 0.f, 0.f, 0.f), cl = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -166,9 +201,9 @@ __device__ __forceinline__ void backward_block(uint32_t W, uint32_t H, uint32_t 
         }
         // Stage the batch in this wave's LDS slots (as in the forward).
         __builtin_amdgcn_wave_barrier();
-        s_spl[wv][lane].co = co;
-        s_spl[wv][lane].cl = make_float4(cl.x, cl.y, cl.z, __uint_as_float(gid));
-        s_spl[wv][lane].p = p;
+        L.spl[lane].co = co;
+        L.spl[lane].cl = make_float4(cl.x, cl.y, cl.z, __uint_as_float(gid));
+        L.spl[lane].p = p;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -177,7 +212,7 @@ __device__ __forceinline__ void backward_block(uint32_t W, uint32_t H, uint32_t 
             const int j = 63 - (int)__builtin_clzll(lm);
             lm &= ~(1ull << j);
             const uint32_t pos = (uint32_t)b - start + (uint32_t)j + 1;
-            const SplatB s = s_spl[wv][j];
+            const SplatB s = L.spl[j];
             const float A = s.co.x, B = s.co.y, C = s.co.z, op = s.co.w;  // A..C: power only
             const float cr = s.cl.x, cg = s.cl.y, cb = s.cl.z;
             const float dx = s.p.x - fx, dy = s.p.y - fy;
@@ -188,39 +223,35 @@ __device__ __forceinline__ void backward_block(uint32_t W, uint32_t H, uint32_t 
             const float alpha = fminf(kAlphaClamp, oe);
             // blended in the forward: the forward's skip tests, as predicates
             const bool hit = (pos <= lst) & !(power > 0.0f) & !(power < -5.55f) & !(alpha < kAlphaMin);
-            // per-splat sums: moments of g = dL/dpower (g dx, g dy, g dx^2,
-            // g dx dy, g dy^2; the conic and position gradients follow from
-            // them per splat), dL/d opacity, dL/d(r, g); dL/db apart.
-            // Every lane evaluates the terms; lanes that did not blend this
-            // splat (or sit on the alpha clamp) select zero multipliers.
-            // alpha <= 0.99 keeps the recovered T finite in every lane.
             // Undo the blend (v_rcp_f32: 1 ulp, no IEEE division sequence);
-            // formed before the skip test so that the colour is read from
-            // LDS together with the rest of the record.
+            // alpha <= 0.99 keeps the recovered T finite in every lane.
             const float r = __builtin_amdgcn_rcpf(1.0f - alpha);
             const float Tb = T * r;
             const float gc = __builtin_fmaf(cr, gr, __builtin_fmaf(cg, gg, cb * gb));
             const float gal = __builtin_fmaf(Tb, gc, -(r * GS));
             asm volatile("" ::"v"(gal));  // keeps gal (and the colour read) ahead of the branch
             if (ballot(hit) == 0) continue;
+            // dL/dpower (zero on the 0.99 clamp, where alpha does not depend
+            // on the power) and the blend weight; both zero where not blended
             const bool grad_ok = hit & (oe <= kAlphaClamp);
             const float w = hit ? alpha * Tb : 0.0f;
             const float gpow = grad_ok ? alpha * gal : 0.0f;
-            const float gdx = gpow * dx, gdy = gpow * dy;
-            const float a[8] = {gdx, gdy, gdx * dx, gdx * dy, gdy * dy, grad_ok ? e * gal : 0.0f, w * gr, w * gg};
-            const float a_b = w * gb;
             // state before this splat where it was blended (w = 0 elsewhere)
             T = hit ? Tb : T;
             GS = __builtin_fmaf(w, gc, GS);
-            float s8, sb;
-            fold9(a, a_b, s8, sb);
-            const uint32_t g = __float_as_uint(s.cl.w);
-            if (g < splats) {
-                float* dst = grad2d + (size_t)g * 12;
-                // one instruction: lanes 0, 8, .., 56 add the folded 8, lane 63 the 9th
-                const bool b8 = lane == 63;
-                if ((lane & 7) == 0 || b8) unsafeAtomicAdd(dst + (b8 ? 8u : tag), b8 ? sb : s8);
+            L.pair[n][lane ^ (n << 3)] = make_float2(gpow, w);
+            slots |= (uint64_t)j << (6 * n);
+            if (++n == (uint32_t)kBwdChunk) {
+                backward_flush(L, n, slots, x0, y0, splats, grad2d);
+                n = 0;
+                slots = 0;
             }
+        }
+        // the next batch overwrites the staged records the slots refer to
+        if (n) {
+            backward_flush(L, n, slots, x0, y0, splats, grad2d);
+            n = 0;
+            slots = 0;
         }
     }
 }
