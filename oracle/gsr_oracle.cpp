@@ -444,6 +444,50 @@ inline uint64_t fnv(uint64_t h, uint64_t v) {
 // same smooth branch of the renderer.
 // evals (optional): pixel-splat evaluations performed under the
 // early-termination rules (the raster's algorithmic VALU work).
+// One pixel (integer coordinates fxp, fyp) blended over `len` splats in
+// order; splat(k) gives the k-th splat's conic (A, B, C), opacity, pixel
+// position and colour, id(k) its index.  Results in (c0, c1, c2), Tr,
+// lastc (1-based position of the last blended splat); h hashes the pixel's
+// discrete decisions, ev counts evaluations.
+template <typename T> struct SplatV {
+    T A, B, C, op, x, y, r, g, b;
+};
+template <typename T, typename Splat, typename Id>
+inline void blend_pixel(T fxp, T fyp, uint32_t len, Splat splat, Id id, T& c0, T& c1, T& c2, T& Tr,
+                        uint32_t& lastc, uint64_t& h, uint64_t& ev) {
+    Tr = 1;
+    c0 = c1 = c2 = 0;
+    lastc = 0;
+    for (uint32_t i = 0; i < len; ++i) {
+        ++ev;
+        const SplatV<T> s = splat(i);
+        const T dx = s.x - fxp, dy = s.y - fyp;
+        // power = -0.5 (A dx^2 + C dy^2) - B dx dy  (calculatePower :113-117)
+        const T q = M<T>::fma(s.A * dx, dx, (s.C * dy) * dy);
+        const T power = M<T>::fma(T(-0.5), q, -((s.B * dx) * dy));
+        if (!M<T>::in_blend_range(power)) continue;
+        const T alpha = std::min(T(0.99f), s.op * M<T>::ex_blend(power));
+        if (alpha < T(1.0f / 255.0f)) continue;
+        const T testT = Tr * (T(1) - alpha);
+        if (testT < T(0.0001f)) {
+            h = fnv(h, 0xFFFFFFFFull + i);
+            break;
+        }
+        h = fnv(h, ((uint64_t)id(i) << 1) | (s.op * M<T>::ex_blend(power) > T(0.99f) ? 1u : 0u));
+        const T wgt = alpha * Tr;
+        c0 = M<T>::fma(s.r, wgt, c0);
+        c1 = M<T>::fma(s.g, wgt, c1);
+        c2 = M<T>::fma(s.b, wgt, c2);
+        Tr = testT;
+        lastc = i + 1;
+    }
+}
+
+// sig (optional, per pixel): hash of the pixel's discrete decisions (which
+// splats it blends and where it stops), to tell when two frames lie on the
+// same smooth branch of the renderer.
+// evals (optional): pixel-splat evaluations performed under the
+// early-termination rules (the raster's algorithmic VALU work).
 template <typename T>
 void raster_all(uint32_t W, uint32_t H, uint32_t tile, const uint32_t* ranges, const uint32_t* vals,
                 const T* conic, const T* xy, const T* rgb, T* out, T* tfinal, uint32_t* last,
@@ -455,39 +499,20 @@ void raster_all(uint32_t W, uint32_t H, uint32_t tile, const uint32_t* ranges, c
     for (int64_t t = 0; t < ntile; ++t) {
         const uint32_t tx = (uint32_t)(t % TXn), ty = (uint32_t)(t / TXn);
         const uint32_t start = ranges[t * 2], end = ranges[t * 2 + 1];
+        auto splat = [&](uint32_t k) {
+            const uint32_t g = vals[start + k];
+            return SplatV<T>{conic[g * 4], conic[g * 4 + 1], conic[g * 4 + 2], conic[g * 4 + 3], xy[g * 2],
+                             xy[g * 2 + 1], rgb[g * 4], rgb[g * 4 + 1], rgb[g * 4 + 2]};
+        };
+        auto id = [&](uint32_t k) { return vals[start + k]; };
         for (uint32_t ly = 0; ly < tile; ++ly)
             for (uint32_t lx = 0; lx < tile; ++lx) {
                 const uint32_t x = tx * tile + lx, y = ty * tile + ly;
                 if (x >= W || y >= H) continue;
-                const T fxp = T(x), fyp = T(y);
-                T Tr = 1, c0 = 0, c1 = 0, c2 = 0;
-                uint32_t lastc = 0;
+                T Tr, c0, c1, c2;
+                uint32_t lastc;
                 uint64_t h = 1469598103934665603ull;
-                for (uint32_t i = start; i < end; ++i) {
-                    ++ev;
-                    const uint32_t g = vals[i];
-                    const T A = conic[g * 4], B = conic[g * 4 + 1], C = conic[g * 4 + 2];
-                    const T op = conic[g * 4 + 3];
-                    const T dx = xy[g * 2] - fxp, dy = xy[g * 2 + 1] - fyp;
-                    // power = -0.5 (A dx^2 + C dy^2) - B dx dy  (calculatePower :113-117)
-                    const T q = M<T>::fma(A * dx, dx, (C * dy) * dy);
-                    const T power = M<T>::fma(T(-0.5), q, -((B * dx) * dy));
-                    if (!M<T>::in_blend_range(power)) continue;
-                    const T alpha = std::min(T(0.99f), op * M<T>::ex_blend(power));
-                    if (alpha < T(1.0f / 255.0f)) continue;
-                    const T testT = Tr * (T(1) - alpha);
-                    if (testT < T(0.0001f)) {
-                        h = fnv(h, 0xFFFFFFFFull + i);
-                        break;
-                    }
-                    h = fnv(h, ((uint64_t)vals[i] << 1) | (op * M<T>::ex_blend(power) > T(0.99f) ? 1u : 0u));
-                    const T wgt = alpha * Tr;
-                    c0 = M<T>::fma(rgb[g * 4], wgt, c0);
-                    c1 = M<T>::fma(rgb[g * 4 + 1], wgt, c1);
-                    c2 = M<T>::fma(rgb[g * 4 + 2], wgt, c2);
-                    Tr = testT;
-                    lastc = i - start + 1;
-                }
+                blend_pixel<T>(T(x), T(y), end > start ? end - start : 0, splat, id, c0, c1, c2, Tr, lastc, h, ev);
                 const size_t p = (size_t)y * W + x;
                 out[p] = c0;
                 out[(size_t)W * H + p] = c1;
@@ -917,6 +942,149 @@ void backward_f64(const float* packed, uint32_t n, const Cam& cam, uint32_t tile
                                 decisions_f32 ? &flags32[g] : nullptr);
 }
 
+// Central differences of L = sum(dLdC * C) of the FP64 frame for every
+// parameter of the given splats, each perturbed frame rendered only over the
+// pixels of the tiles the splat's rectangle covers (with the base parameters
+// or either perturbation) where either perturbed splat can blend (power <= 0
+// and alpha >= 1/255, tested with a widened threshold): at every other pixel
+// both frames blend the same splats in the same order, so the difference of
+// the rendered pixels' L is the whole frame's.  The window's instance lists are the base
+// frame's (emit order, stable by depth16) with the perturbed splat removed
+// and re-inserted at its key.  fd[k*59+ch]; same[k*59+ch] = 1 when both
+// perturbed windows take the same discrete decisions (blends, stops, clamps,
+// the splat's tile rectangle and depth key).
+void fd_window_f64(const float* packed, uint32_t n, const Cam& cam, uint32_t tile, const double* dLdC,
+                   const uint32_t* splats, uint32_t count, float eps, double* fd, uint8_t* same) {
+    const uint32_t W = (uint32_t)cam.texture_size[0], H = (uint32_t)cam.texture_size[1];
+    const uint32_t TXn = (W + tile - 1) / tile, TYn = (H + tile - 1) / tile;
+    const size_t N = n, ntile = (size_t)TXn * TYn;
+    std::vector<uint32_t> tiles(N), meta(N * 4);
+    std::vector<double> conic(N * 4), xy(N * 2), rgb(N * 4);
+    preprocess_all<double>(packed, n, cam, tile, tiles.data(), meta.data(), conic.data(), xy.data(), rgb.data());
+    // base per-tile lists: splats in index order, then stable by depth16
+    std::vector<uint64_t> start(ntile + 1, 0);
+    auto each_tile = [&](const uint32_t* m, auto fn) {
+        const uint32_t x0 = m[0] & 0xFFFF, y0 = m[0] >> 16, x1 = m[1] & 0xFFFF, y1 = m[1] >> 16;
+        for (uint32_t ty = y0; ty <= y1; ++ty)
+            for (uint32_t tx = x0; tx <= x1; ++tx) fn((size_t)ty * TXn + tx);
+    };
+    for (size_t g = 0; g < N; ++g)
+        if (tiles[g]) each_tile(&meta[g * 4], [&](size_t t) { ++start[t + 1]; });
+    for (size_t t = 0; t < ntile; ++t) start[t + 1] += start[t];
+    std::vector<uint32_t> list(start[ntile]);
+    {
+        std::vector<uint64_t> cur(start.begin(), start.end() - 1);
+        for (size_t g = 0; g < N; ++g)
+            if (tiles[g]) each_tile(&meta[g * 4], [&](size_t t) { list[cur[t]++] = (uint32_t)g; });
+    }
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t t = 0; t < (int64_t)ntile; ++t)
+        std::stable_sort(list.begin() + start[t], list.begin() + start[t + 1],
+                         [&](uint32_t a, uint32_t b) { return meta[a * 4 + 2] < meta[b * 4 + 2]; });
+    // packed block offset of planar channel ch of splat g in an array of nn splats
+    auto offset = [](uint32_t nn, int ch, uint32_t g) -> size_t {
+        const size_t M = nn;
+        if (ch < 3) return (size_t)g * 3 + ch;
+        if (ch < 6) return M * 3 + (size_t)g * 3 + (ch - 3);
+        if (ch < 10) return M * 6 + (size_t)g * 4 + (ch - 6);
+        if (ch < 13) return M * 10 + (size_t)g * 3 + (ch - 10);
+        if (ch < 14) return M * 13 + g;
+        return M * 14 + (size_t)g * 45 + (ch - 14);
+    };
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t job = 0; job < (int64_t)count * 59; ++job) {
+        const uint32_t g = splats[job / 59];
+        const int ch = (int)(job % 59);
+        float one[59];
+        for (int c = 0; c < 59; ++c) one[offset(1, c, 0)] = packed[offset(n, c, g)];
+        const float v = one[offset(1, ch, 0)];
+        const float vs[2] = {v + eps, v - eps};
+        Pre<double> side[2];
+        for (int k = 0; k < 2; ++k) {
+            one[offset(1, ch, 0)] = vs[k];
+            preprocess_one<double>(Packed{one, 1}, 0, cam, tile, side[k]);
+        }
+        // window: union of the base and perturbed tile rectangles
+        uint32_t wx0 = ~0u, wy0 = ~0u, wx1 = 0, wy1 = 0;
+        auto grow = [&](const uint32_t* m) {
+            wx0 = std::min(wx0, m[0] & 0xFFFF); wy0 = std::min(wy0, m[0] >> 16);
+            wx1 = std::max(wx1, m[1] & 0xFFFF); wy1 = std::max(wy1, m[1] >> 16);
+        };
+        if (tiles[g]) grow(&meta[(size_t)g * 4]);
+        for (int k = 0; k < 2; ++k)
+            if (side[k].tiles) grow(side[k].meta);
+        if (wx0 == ~0u) {  // culled in all three frames
+            fd[job] = 0.0;
+            same[job] = 1;
+            continue;
+        }
+        // pixels where either perturbed splat can blend: 0 <= q <= 2 ln(255 op), widened
+        auto reach = [&](const Pre<double>& P, double x, double y) {
+            if (!P.tiles) return false;
+            const double op = P.conic[3];
+            if (!(op * 255.0 > 1.0)) return false;
+            const double dx = P.xy[0] - x, dy = P.xy[1] - y;
+            const double q = P.conic[0] * dx * dx + 2.0 * P.conic[1] * dx * dy + P.conic[2] * dy * dy;
+            return !(q > 1.02 * 2.0 * std::log(255.0 * op) + 0.1);
+        };
+        double L[2];
+        uint64_t sig[2];
+        std::vector<uint32_t> loc;
+        for (int k = 0; k < 2; ++k) {
+            const Pre<double>& P = side[k];
+            const uint32_t px0 = P.meta[0] & 0xFFFF, py0 = P.meta[0] >> 16, px1 = P.meta[1] & 0xFFFF,
+                           py1 = P.meta[1] >> 16;
+            uint64_t h = 1469598103934665603ull;
+            h = fnv(h, P.tiles ? ((uint64_t)P.meta[0] << 32 | P.meta[1]) : 0);
+            h = fnv(h, P.tiles ? P.meta[2] : 0xFFFFFFFFull);
+            double acc = 0.0;
+            uint64_t ev = 0;
+            for (uint32_t ty = wy0; ty <= wy1; ++ty)
+                for (uint32_t tx = wx0; tx <= wx1; ++tx) {
+                    const size_t t = (size_t)ty * TXn + tx;
+                    loc.clear();
+                    const bool in = P.tiles && tx >= px0 && tx <= px1 && ty >= py0 && ty <= py1;
+                    bool placed = !in;
+                    for (uint64_t i = start[t]; i < start[t + 1]; ++i) {
+                        const uint32_t o = list[i];
+                        if (o == g) continue;
+                        if (!placed && (meta[o * 4 + 2] > P.meta[2] || (meta[o * 4 + 2] == P.meta[2] && o > g))) {
+                            loc.push_back(g);
+                            placed = true;
+                        }
+                        loc.push_back(o);
+                    }
+                    if (!placed) loc.push_back(g);
+                    auto splat = [&](uint32_t q) {
+                        const uint32_t o = loc[q];
+                        if (o == g)
+                            return SplatV<double>{P.conic[0], P.conic[1], P.conic[2], P.conic[3], P.xy[0], P.xy[1],
+                                                  P.rgb[0], P.rgb[1], P.rgb[2]};
+                        return SplatV<double>{conic[o * 4], conic[o * 4 + 1], conic[o * 4 + 2], conic[o * 4 + 3],
+                                              xy[o * 2], xy[o * 2 + 1], rgb[o * 4], rgb[o * 4 + 1], rgb[o * 4 + 2]};
+                    };
+                    auto id = [&](uint32_t q) { return loc[q]; };
+                    for (uint32_t ly = 0; ly < tile; ++ly)
+                        for (uint32_t lx = 0; lx < tile; ++lx) {
+                            const uint32_t x = tx * tile + lx, y = ty * tile + ly;
+                            if (x >= W || y >= H) continue;
+                            if (!reach(side[0], x, y) && !reach(side[1], x, y)) continue;
+                            double Tr, c0, c1, c2;
+                            uint32_t lastc;
+                            blend_pixel<double>(double(x), double(y), (uint32_t)loc.size(), splat, id, c0, c1, c2, Tr,
+                                                lastc, h, ev);
+                            const size_t pix = (size_t)y * W + x;
+                            acc += c0 * dLdC[pix] + c1 * dLdC[(size_t)W * H + pix] + c2 * dLdC[(size_t)2 * W * H + pix];
+                        }
+                }
+            L[k] = acc;
+            sig[k] = h;
+        }
+        fd[job] = (L[0] - L[1]) / ((double)vs[0] - (double)vs[1]);
+        same[job] = sig[0] == sig[1] ? 1 : 0;
+    }
+}
+
 // FP32 replay of the raster adjoint with the same steps as the GPU kernel
 // (T recovered with 1 / (1 - alpha), the dL/dRGB-weighted light from behind
 // GS = sum_ch g_ch sum_behind c_j alpha_j T_j, so that
@@ -1161,6 +1329,11 @@ void orc_fd_f64(const float* packed, uint32_t n, const void* cam, uint32_t tile,
             same[job] = sig[0] == sig[1] ? 1 : 0;
         }
     }
+}
+
+void orc_fd_window_f64(const float* packed, uint32_t n, const void* cam, uint32_t tile, const double* dLdC,
+                       const uint32_t* splats, uint32_t count, float eps, double* fd, uint8_t* same) {
+    fd_window_f64(packed, n, *(const Cam*)cam, tile, dLdC, splats, count, eps, fd, same);
 }
 
 void orc_raster_backward_f32(uint32_t W, uint32_t H, uint32_t tile, const uint32_t* ranges, const uint32_t* vals,

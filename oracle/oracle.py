@@ -64,6 +64,9 @@ def lib():
         L.orc_param_mass_f64.argtypes = [_f32p, ctypes.c_uint32, _f32p, _f64p, _f64p]
         L.orc_fd_f64.argtypes = [_f32p, ctypes.c_uint32, _f32p, ctypes.c_uint32, _f64p, _u32p, ctypes.c_uint32,
                                  ctypes.c_float, _f64p, np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")]
+        L.orc_fd_window_f64.argtypes = [_f32p, ctypes.c_uint32, _f32p, ctypes.c_uint32, _f64p, _u32p,
+                                        ctypes.c_uint32, ctypes.c_float, _f64p,
+                                        np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")]
         L.orc_raster_backward_f32.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p, _f32p,
                                               _f32p, _f32p, _f32p, _u32p, _f32p, ctypes.c_uint32, _f64p]
         L.orc_preprocess_backward_f64.argtypes = [_f32p, ctypes.c_uint32, _f32p, _f64p, _f64p]
@@ -222,6 +225,20 @@ def finite_differences(packed, n, uni, dLdC, splats, eps=1e-4, tile=16):
     same = np.zeros(k * 59, np.uint8)
     lib().orc_fd_f64(np.ascontiguousarray(packed, np.float32), n, np.ascontiguousarray(uni, np.float32), tile,
                      np.ascontiguousarray(dLdC, np.float64).ravel(), splats, k, eps, fd, same)
+    return fd.reshape(k, 59).T, same.reshape(k, 59).T.astype(bool)
+
+
+def finite_differences_window(packed, n, uni, dLdC, splats, eps=1e-4, tile=16):
+    """finite_differences with each perturbed frame rendered only over the pixels of the
+    splat's tiles (base or perturbed) where either perturbed splat can blend: the same
+    central differences at a cost set by the splat's footprint, not the frame.
+    Returns (fd [59, k], same_branch [59, k])."""
+    splats = np.ascontiguousarray(splats, np.uint32)
+    k = splats.size
+    fd = np.zeros(k * 59, np.float64)
+    same = np.zeros(k * 59, np.uint8)
+    lib().orc_fd_window_f64(np.ascontiguousarray(packed, np.float32), n, np.ascontiguousarray(uni, np.float32), tile,
+                            np.ascontiguousarray(dLdC, np.float64).ravel(), splats, k, eps, fd, same)
     return fd.reshape(k, 59).T, same.reshape(k, 59).T.astype(bool)
 
 

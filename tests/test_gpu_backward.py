@@ -63,7 +63,7 @@ def _compare(name, got, ref, mass, axis_groups, axis):
         bound = 1e-3 * np.abs(b) + 1e-4 * m
         ratio = np.abs(a - b) / np.maximum(bound, 1e-30)
         _say(f"{name}/{gname}: norm rel err {rel:.2e}; error/bound p99.9 {np.quantile(ratio, 0.999):.3f} "
-             f"max {ratio.max():.3f}")
+             f"max {ratio.max():.3f}; share above the bound {(ratio > 1.0).mean():.2e}")
         assert rel < 1e-3, (gname, rel)
         assert np.quantile(ratio, 0.999) <= 1.0, (gname, np.quantile(ratio, 0.999))
 

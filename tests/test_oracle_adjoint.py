@@ -1,7 +1,8 @@
 """Pins of the FP64 adjoint (oracle.backward) — CPU only.
 
 1. Central finite differences (eps 1e-4, FP64 forward) on 256 randomly chosen
-   splats, every one of the 59 parameters.  A component whose +eps and -eps
+   splats, every one of the 59 parameters, on a 384-splat scene and on the
+   1M-splat 1080p scene of the benchmark.  A component whose +eps and -eps
    frames differ in any discrete decision (tile membership, alpha cutoff,
    saturation stop, colour clamp) sits on a cliff of the renderer and is
    excluded; the excluded share is bounded.
@@ -47,6 +48,46 @@ def test_adjoint_matches_central_differences(orc):
     # every parameter group is exercised on the same branch
     for sl in (slice(0, 3), slice(3, 6), slice(6, 10), slice(10, 13), slice(13, 14), slice(14, 59)):
         assert same[sl].sum() > 0.8 * same[sl].size
+
+
+def test_adjoint_matches_central_differences_on_the_1m_scene(orc):
+    """The same check on the 1,000,000-splat 1920x1080 scene of the benchmark
+    (dL/dRGB ~ N(0, 1)): central differences (eps 1e-4, FP64 frames) of 256
+    randomly chosen visible splats, every one of their 59 parameters, against
+    the FP64 adjoint of the whole frame.  Each perturbed frame is rendered only
+    over the pixels where the splat can blend (oracle.finite_differences_window,
+    checked against whole-frame differences below): every other pixel is the
+    same in both frames.
+    Position components cross discrete decisions most often (a shift of
+    1.5e-2 px moves some pixel of the splat's footprint across the 1/255
+    cutoff or a stop), so their branch share is bounded separately."""
+    sc, cam = scenes.workload_1m_1080p()
+    u = cam.uniform(sc.count)
+    n = sc.count
+    rng = np.random.default_rng(2)
+    dLdC = rng.normal(size=(3, cam.height, cam.width))
+    grad, _, _ = orc.backward(sc.packed(), n, u, dLdC, decisions_f32=False)
+    pre = orc.preprocess(sc.packed(), n, u, fp64=True)
+    chosen = np.sort(rng.choice(np.nonzero(pre["tiles"] > 0)[0], size=256, replace=False))
+    fd, same = orc.finite_differences_window(sc.packed(), n, u, dLdC, chosen, eps=1e-4)
+    an = grad[:, chosen]
+    errs = (np.abs(fd - an) / np.maximum(np.maximum(np.abs(an), np.abs(fd)), 1e-4))[same]
+    assert same.mean() > 0.95, same.mean()
+    assert np.quantile(errs, 0.999) < 1e-3 and errs.max() < 1e-2, (errs.max(), np.quantile(errs, 0.999))
+    assert same[0:3].mean() > 0.5
+    for sl in (slice(3, 6), slice(6, 10), slice(10, 13), slice(13, 14), slice(14, 59)):
+        assert same[sl].mean() > 0.95
+
+
+def test_windowed_differences_equal_whole_frame_differences(orc):
+    sc, cam = _scene()
+    u = cam.uniform(sc.count)
+    dLdC = np.random.default_rng(3).normal(size=(3, cam.height, cam.width))
+    chosen = np.arange(0, sc.count, 12)
+    a, sa = orc.finite_differences(sc.packed(), sc.count, u, dLdC, chosen)
+    b, sb = orc.finite_differences_window(sc.packed(), sc.count, u, dLdC, chosen)
+    assert np.array_equal(sa, sb)
+    assert np.abs(a - b).max() <= 1e-8 * max(1.0, np.abs(a).max())
 
 
 def test_fp32_decisions_agree_with_fp64_decisions(orc):
