@@ -607,30 +607,33 @@ __global__ __launch_bounds__(256) void k_preprocess_backward(const float* __rest
         grad2d[(size_t)g * 3 + 1] = make_float4(Gv[4], G1.y, G1.z, G1.w);
     }
     SplatShape s;
-    float gp[3], gsym[3][3], gch[3], basis[15];
+    float gp[3], gsym[3][3], gch[3], basis[15], gs[7], gop = 0.0f;
     bool ok = any;
     if (ok) {
         splat_shape(par, s);
         ok = view_adjoint(par, s, cam, Gv, gp, gsym, gch, basis);
     }
-    if (!ok) {
-        for (int ch = 0; ch < (int)kParams; ++ch) grad[(size_t)ch * N + g] = 0.0f;
-        return;
+    if (ok) {
+        shape_adjoint(s, gsym, gs);
+        const float op = opacity_of(par);
+        gop = Gv[5] * op * (1.0f - op);
     }
-    float gs[7];
-    shape_adjoint(s, gsym, gs);
+    // Every lane stores through the same instructions, zeros where the splat
+    // has no gradient: stores of one line from two branches far apart in
+    // time let a partly written line leave the cache and be written twice.
+    // (The selects keep the arrays unset on that path: no registers for zeros.)
 #pragma unroll
-    for (int k = 0; k < 7; ++k) grad[(size_t)(CH_SCALE + k) * N + g] = gs[k];  // scale, then rotation
-    const float op = opacity_of(par);
-    grad[(size_t)CH_OPACITY * N + g] = Gv[5] * op * (1.0f - op);
+    for (int k = 0; k < 7; ++k) grad[(size_t)(CH_SCALE + k) * N + g] = ok ? gs[k] : 0.0f;  // scale, rotation
+    grad[(size_t)CH_OPACITY * N + g] = gop;
 #pragma unroll
     for (int ch = 0; ch < 3; ++ch) {
-        grad[(size_t)(CH_DC + ch) * N + g] = SH_C0 * gch[ch];
+        grad[(size_t)(CH_DC + ch) * N + g] = ok ? SH_C0 * gch[ch] : 0.0f;
 #pragma unroll
-        for (int k = 0; k < 15; ++k) grad[(size_t)(CH_REST + 15 * ch + k) * N + g] = basis[k] * gch[ch];
+        for (int k = 0; k < 15; ++k)
+            grad[(size_t)(CH_REST + 15 * ch + k) * N + g] = ok ? basis[k] * gch[ch] : 0.0f;
     }
 #pragma unroll
-    for (int c = 0; c < 3; ++c) grad[(size_t)(CH_POS + c) * N + g] = gp[c];
+    for (int c = 0; c < 3; ++c) grad[(size_t)(CH_POS + c) * N + g] = ok ? gp[c] : 0.0f;
 }
 
 // The part of SplatShape the per-view adjoint reads.
@@ -714,20 +717,21 @@ __global__ __launch_bounds__(256) void k_preprocess_backward_batch(const float* 
             for (int k = 0; k < 15; ++k) srest[15 * ch + k] = __builtin_fmaf(basis[k], gch[ch], srest[15 * ch + k]);
         }
     }
-    if (!touched) {
-        for (int ch = 0; ch < (int)kParams; ++ch) grad[(size_t)ch * N + g] = 0.0f;
-        return;
+    // an untouched splat keeps its zero sums and gets zeros through the same
+    // stores as the others (one store instruction per channel and wave)
+    float gs[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, gop = 0.0f;
+    if (touched) {
+        const float* Pe = opaque(P);  // the shape again, not kept live through the loop
+        SplatShape s;
+        splat_shape(ParGlobal{Pe, N, g}, s);
+        const float sym[3][3] = {{ss[0], ss[1], ss[2]}, {ss[1], ss[3], ss[4]}, {ss[2], ss[4], ss[5]}};
+        shape_adjoint(s, sym, gs);
+        const float op = opacity_of(ParGlobal{Pe, N, g});
+        gop = sop * op * (1.0f - op);
     }
-    const float* Pe = opaque(P);  // the shape again, not kept live through the loop
-    SplatShape s;
-    splat_shape(ParGlobal{Pe, N, g}, s);
-    const float sym[3][3] = {{ss[0], ss[1], ss[2]}, {ss[1], ss[3], ss[4]}, {ss[2], ss[4], ss[5]}};
-    float gs[7];
-    shape_adjoint(s, sym, gs);
 #pragma unroll
     for (int k = 0; k < 7; ++k) grad[(size_t)(CH_SCALE + k) * N + g] = gs[k];  // scale, then rotation
-    const float op = opacity_of(ParGlobal{Pe, N, g});
-    grad[(size_t)CH_OPACITY * N + g] = sop * op * (1.0f - op);
+    grad[(size_t)CH_OPACITY * N + g] = gop;
 #pragma unroll
     for (int ch = 0; ch < 3; ++ch) {
         grad[(size_t)(CH_DC + ch) * N + g] = SH_C0 * sch[ch];
