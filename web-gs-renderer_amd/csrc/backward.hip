@@ -115,7 +115,8 @@ __device__ __forceinline__ void backward_flush(BwdWaveLds& L, uint32_t n, uint64
         v[k] = t;
     }
     // opacity: sum of e gal over the pixels off the 0.99 clamp = sum gpow / op
-    v[5] = v[5] / op;
+    // (v_rcp_f32, 1 ulp, instead of an IEEE division sequence)
+    v[5] = v[5] * __builtin_amdgcn_rcpf(op);
     // lane col of a splat's 8 lanes adds total col (one atomic instruction
     // whose lanes address 8 consecutive floats per splat), the first lane
     // the ninth
@@ -123,6 +124,9 @@ __device__ __forceinline__ void backward_flush(BwdWaveLds& L, uint32_t n, uint64
     const float s01 = b0 ? v[1] : v[0], s23 = b0 ? v[3] : v[2], s45 = b0 ? v[5] : v[4], s67 = b0 ? v[7] : v[6];
     const float s03 = b1 ? s23 : s01, s47 = b1 ? s67 : s45;
     const float mine = b2 ? s47 : s03;
+    // the totals are formed by every lane before the branch below: sunk into
+    // it, the last DPP step could not fold into its add
+    asm volatile("" ::"v"(mine), "v"(v[8]));
     // the collected slots are reused by the next chunk
     __builtin_amdgcn_wave_barrier();
     if (c < n && gid < splats) {
