@@ -152,3 +152,40 @@ def test_edge_frame_backward(torch_cuda, ctx, orc, tile):
         rel = np.linalg.norm(a - b) / np.linalg.norm(b)
         _say(f"params/{gname}: norm rel {rel:.2e}")
         assert rel < 1e-3, (gname, rel)
+
+
+@pytest.mark.parametrize("tile", [8, 16, 32])
+def test_raster_never_culls_an_indefinite_conic(torch_cuda, ctx, orc, tile):
+    """A conic with A < 0 and C < 0 (what a covariance whose FP32 determinant
+    rounds below zero gives: preprocess.wgsl:103-110 has no guard) is
+    indefinite, and pixels off its axes still meet q in (0, q_max].  Splat 0:
+    conic (-0.04, 0.11, -0.04), opacity 0.9, centre (0.5, 0.5): pixel (8, 8)
+    has q = 7.875 (alpha 0.018) and is blended, but along each edge of the
+    block [8, 15]^2 q is concave, so its stationary point is the edge's
+    maximum (13.3 > q_max = 11.19): a cull that takes the stationary points
+    for the minima would skip the whole block.  The raster must match the
+    oracle (which culls nothing) bit for bit.  Splat 1 is an ordinary splat
+    behind it."""
+    W = H = 32
+    conic = np.array([[-0.04, 0.11, -0.04, 0.9], [0.05, 0.01, 0.04, 0.6]], np.float32)
+    xy = np.array([[0.5, 0.5], [20.0, 12.0]], np.float32)
+    rgb = np.array([[1.0, 0.5, 0.25, 0.0], [0.2, 0.7, 0.3, 0.0]], np.float32)
+    TX = (W + tile - 1) // tile
+    tiles = TX * TX
+    # every tile holds both splats in depth order (0 in front)
+    vs = np.tile(np.array([0, 1], np.uint32), tiles)
+    ranges = np.stack([np.arange(tiles) * 2, np.arange(tiles) * 2 + 2], 1).astype(np.uint32).ravel()
+    img_o, T_o, last_o = orc.raster(W, H, tile, ranges, vs, conic, xy, rgb)
+    assert last_o[8, 8] >= 1 and img_o[0, 8:16, 8:16].max() > 0.01   # the block is blended
+    d = {k: to_dev(torch_cuda, np.ascontiguousarray(v).ravel()) for k, v in (("conic", conic), ("xy", xy), ("rgb", rgb))}
+    out = empty_dev(torch_cuda, 3 * W * H, np.float32)
+    T = empty_dev(torch_cuda, W * H, np.float32)
+    last = empty_dev(torch_cuda, W * H, np.uint32)
+    d_r, d_v = to_dev(torch_cuda, ranges), to_dev(torch_cuda, vs)
+    torch_cuda.cuda.synchronize()
+    _say(f"indefinite conic raster tile={tile}")
+    ctx.rasterize(W, H, tile, 2, vs.size, d_r, d_v, d["conic"], d["xy"], d["rgb"], out, T, last)
+    img = to_host(out, np.float32).reshape(3, H, W)
+    assert np.array_equal(img, img_o)
+    assert np.array_equal(to_host(T, np.float32).reshape(H, W), T_o)
+    assert np.array_equal(to_host(last, np.uint32).reshape(H, W), last_o)

@@ -211,11 +211,14 @@ __device__ __forceinline__ void backward_block(uint32_t W, uint32_t H, uint32_t 
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        // live splats back to front (highest lane first)
+        // live splats back to front (highest lane first); base: 1 + the
+        // batch's position in the range, as a scalar, so that a splat's
+        // position is a scalar add
+        const uint32_t base = (uint32_t)__builtin_amdgcn_readfirstlane((int)((uint32_t)b - start + 1));
         for (uint64_t lm = ballot(live); lm != 0;) {
             const int j = 63 - (int)__builtin_clzll(lm);
             lm &= ~(1ull << j);
-            const uint32_t pos = (uint32_t)b - start + (uint32_t)j + 1;
+            const uint32_t pos = base + (uint32_t)j;
             const SplatB s = L.spl[j];
             const float A = s.co.x, B = s.co.y, C = s.co.z, op = s.co.w;  // A..C: power only
             const float cr = s.cl.x, cg = s.cl.y, cb = s.cl.z;

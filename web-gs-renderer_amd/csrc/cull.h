@@ -17,14 +17,17 @@
 
 namespace gsr {
 
-// q over the edge with dx fixed, dy in [lo, hi]: minimum at dy = -B dx / C.
-__device__ __forceinline__ float edge_min_dx(float A, float B, float C, float dx, float lo, float hi) {
-    const float dy = fminf(fmaxf(-B * dx / C, lo), hi);
+// q over the edge with dx fixed, dy in [lo, hi]: minimum at dy = -B dx / C
+// (nbc = -B / C).  The stationary point comes from v_rcp_f32 (1 ulp): q at a
+// point off the exact one by a relative 1e-7 exceeds the minimum by about
+// C (1e-7 dy)^2, far inside the threshold's widening.
+__device__ __forceinline__ float edge_min_dx(float A, float B, float C, float nbc, float dx, float lo, float hi) {
+    const float dy = fminf(fmaxf(nbc * dx, lo), hi);
     return A * dx * dx + 2.0f * B * dx * dy + C * dy * dy;
 }
 
-__device__ __forceinline__ float edge_min_dy(float A, float B, float C, float dy, float lo, float hi) {
-    const float dx = fminf(fmaxf(-B * dy / A, lo), hi);
+__device__ __forceinline__ float edge_min_dy(float A, float B, float C, float nba, float dy, float lo, float hi) {
+    const float dx = fminf(fmaxf(nba * dy, lo), hi);
     return A * dx * dx + 2.0f * B * dx * dy + C * dy * dy;
 }
 
@@ -36,15 +39,23 @@ __device__ __forceinline__ float reach_qmax(float op) {
 }
 
 // True unless the splat provably misses every pixel centre of [xa, xb] x [ya, yb].
+// With A > 0 and C > 0, q is convex along every edge of the rectangle and
+// has no interior minimum unless it is positive definite, whose minimum is
+// the centre (tested first): the smallest edge minimum is the rectangle's.
+// A conic with A <= 0 or C <= 0 (a covariance whose FP32 determinant came
+// out <= 0: q is then indefinite and may still be in (0, q_max] off its
+// axes) is never culled.
 __device__ __forceinline__ bool reaches_rect(float gx, float gy, float A, float B, float C, float qmax, float xa,
                                              float xb, float ya, float yb) {
     if (!(qmax >= 0.0f)) return false;
+    if (!(A > 0.0f) || !(C > 0.0f)) return true;
     if (gx >= xa && gx <= xb && gy >= ya && gy <= yb) return true;
     const float dxlo = gx - xb, dxhi = gx - xa, dylo = gy - yb, dyhi = gy - ya;
-    float q = edge_min_dx(A, B, C, dxhi, dylo, dyhi);
-    q = fminf(q, edge_min_dx(A, B, C, dxlo, dylo, dyhi));
-    q = fminf(q, edge_min_dy(A, B, C, dyhi, dxlo, dxhi));
-    q = fminf(q, edge_min_dy(A, B, C, dylo, dxlo, dxhi));
+    const float nbc = -B * __builtin_amdgcn_rcpf(C), nba = -B * __builtin_amdgcn_rcpf(A);
+    float q = edge_min_dx(A, B, C, nbc, dxhi, dylo, dyhi);
+    q = fminf(q, edge_min_dx(A, B, C, nbc, dxlo, dylo, dyhi));
+    q = fminf(q, edge_min_dy(A, B, C, nba, dyhi, dxlo, dxhi));
+    q = fminf(q, edge_min_dy(A, B, C, nba, dylo, dxlo, dxhi));
     return !(q > qmax);
 }
 

@@ -140,6 +140,9 @@ __device__ __forceinline__ void raster_block(uint32_t W, uint32_t H, uint32_t x0
         // one wait; the tests are the skip/stop conditions above, in the same
         // order of effect.  colour += c * 0 where nothing is blended leaves
         // it unchanged bit for bit (colours are finite and >= 0).
+        // 1 + the batch's position in the range, as a scalar: the last
+        // contributor's index is then a scalar add per splat
+        const uint32_t base = (uint32_t)__builtin_amdgcn_readfirstlane((int)(b - start + 1));
         for (uint64_t lm = ballot(live); lm != 0; lm &= lm - 1) {
             const int j = (int)__builtin_ctzll(lm);
             const SplatL s = s_spl[wv][j];
@@ -149,15 +152,16 @@ __device__ __forceinline__ void raster_block(uint32_t W, uint32_t H, uint32_t x0
             const float alpha = fminf(kAlphaClamp, s.co.w * exp_blend(power));
             const float tn = T * (1.0f - alpha);
             const bool ok = !done & !(power > 0.0f) & !(power < -5.55f) & !(alpha < kAlphaMin);
-            const bool low = tn < kTransmitMin;
-            done |= ok & low;
-            const bool use = ok & !low;
+            done |= ok & (tn < kTransmitMin);
+            // ok lanes were not done before: blended where still not done
+            // (one compare of tn, the other test a mask operation)
+            const bool use = ok & !done;
             const float w = use ? alpha * T : 0.0f;
             c0 = __builtin_fmaf(s.cl.x, w, c0);
             c1 = __builtin_fmaf(s.cl.y, w, c1);
             c2 = __builtin_fmaf(s.cl.z, w, c2);
             T = use ? tn : T;
-            if (SAVE) lst = use ? b - start + (uint32_t)j + 1 : lst;
+            if (SAVE) lst = use ? base + (uint32_t)j : lst;
         }
     }
     if (inside) {
