@@ -87,6 +87,28 @@ def edge_frame(n=4096, seed=11):
     return sc, cam
 
 
+# A flat pancake (scales e^4.36, e^4.51, e^-12.8) seen almost edge-on: its
+# 2-D covariance is singular but for the 0.3 low-pass, and the FP32
+# determinant a c - b^2 (entries near 1e7) rounds below zero, so the FP32
+# conic has A < 0 and C < 0: an indefinite q that some pixels off its axes
+# still meet in (0, q_max] (preprocess.wgsl:103-110 has no guard).
+INDEFINITE_PANCAKE = dict(pos=(0.17569424, 0.18081267, -3.2771184), lscale=(4.3554664, 4.508484, -12.785446),
+                          rot=(0.20866793, 0.21627764, 0.7852127, 0.54140806), opacity=0.7369, dc=0.5)
+
+
+def indefinite_frame(n=4096, seed=11):
+    """edge_frame(n, seed) plus INDEFINITE_PANCAKE as its last splat."""
+    sc, cam = edge_frame(n, seed)
+    p = INDEFINITE_PANCAKE
+    sc = scenes.Scene(pos=np.concatenate([sc.pos, np.array([p["pos"]], np.float32)]),
+                      lscale=np.concatenate([sc.lscale, np.array([p["lscale"]], np.float32)]),
+                      rot=np.concatenate([sc.rot, np.array([p["rot"]], np.float32)]),
+                      dc=np.concatenate([sc.dc, np.full((1, 3), p["dc"], np.float32)]),
+                      opacity=np.concatenate([sc.opacity, np.array([p["opacity"]], np.float32)]),
+                      rest=np.concatenate([sc.rest, np.zeros((1, 45), np.float32)]))
+    return sc, cam
+
+
 def powers_in_rect(pre, g, W, H, tile=16):
     """FP64 powers of splat g at every pixel of the tiles its rectangle covers
     (NaN elsewhere), from the oracle's FP32 preprocess outputs."""

@@ -189,3 +189,39 @@ def test_raster_never_culls_an_indefinite_conic(torch_cuda, ctx, orc, tile):
     assert np.array_equal(img, img_o)
     assert np.array_equal(to_host(T, np.float32).reshape(H, W), T_o)
     assert np.array_equal(to_host(last, np.uint32).reshape(H, W), last_o)
+
+
+def test_indefinite_conic_frame(torch_cuda, ctx, orc):
+    """The edge frame plus a pancake whose FP32 conic comes out indefinite
+    (A < 0, C < 0; tests/edge_scenes.py): the whole frame bit-exact against
+    the FP32 oracle, and its backward (tile 16) against the FP32 twin on
+    every element."""
+    from edge_scenes import indefinite_frame
+    sc, cam = indefinite_frame()
+    u = cam.uniform(sc.count)
+    W, H, n, tile = cam.width, cam.height, sc.count, 16
+    pre = orc.preprocess(sc.packed(), n, u, tile=tile)
+    assert pre["conic"][n - 1, 0] < 0 and pre["conic"][n - 1, 2] < 0 and pre["tiles"][n - 1] > 0
+    _say(f"indefinite conic frame n={n}")
+    ctx.scene_upload("indef", sc.packed(), n)
+    ctx.scene_select("indef")
+    img = ctx.render(u)
+    ref, _, _, Iref, _ = orc.render(sc.packed(), n, u, use_radix=True)
+    assert ctx.instance_count() == Iref
+    assert np.array_equal(img, ref), np.abs(img - ref).max()
+    ctx.scene_free("indef")
+    dLdC = np.random.default_rng(9).normal(size=(3, H, W))
+    grad, g2, _ = _edge_grads(torch_cuda, ctx, sc, u, dLdC, tile)
+    assert np.isfinite(grad).all() and np.isfinite(g2).all()
+    _, _, _, m2, _ = orc.backward(sc.packed(), n, u, dLdC, tile=tile, decisions_f32=True, want_mass=True)
+    offs, total = orc.scan(np.concatenate([pre["tiles"], [0]]).astype(np.uint32))
+    keys, vals = orc.emit(offs, pre["meta"], n, W, tile, 1 << 40)
+    ks, vs = orc.sort_stable(keys, vals)
+    ranges = orc.tile_ranges(ks, ((W + tile - 1) // tile) * ((H + tile - 1) // tile))
+    _, T, last = orc.raster(W, H, tile, ranges, vs, pre["conic"], pre["xy"], pre["rgb"])
+    twin = orc.raster_backward_f32(W, H, tile, ranges, vs, pre["conic"], pre["xy"], pre["rgb"], T, last,
+                                   np.asarray(dLdC, np.float32), n)
+    assert np.abs(twin[n - 1, :9]).max() > 0   # the pancake is blended and differentiated
+    err = np.abs(g2[:, :9] - twin[:, :9]) / np.maximum(1e-4 * m2[:, :9], 1e-30)
+    _say(f"twin: error / (1e-4 mass) max {err.max():.3f}")
+    assert err.max() <= 1.0

@@ -145,6 +145,25 @@ def test_edge_frame_far_needles_blend_nowhere(orc):
         assert np.array_equal(a, b) and np.array_equal(Ta, Tb), fp64
 
 
+def test_indefinite_pancake_conic(orc):
+    """The planted pancake of tests/edge_scenes.py: in FP32 its 2-D
+    determinant rounds below zero, giving a visible splat with an indefinite
+    conic (A < 0, C < 0) that blends some pixels; in FP64 the same splat's
+    conic is positive definite."""
+    from edge_scenes import edge_frame, indefinite_frame
+    sc, cam = indefinite_frame()
+    u = cam.uniform(sc.count)
+    g = sc.count - 1
+    c32 = orc.preprocess(sc.packed(), sc.count, u)["conic"][g]
+    c64 = orc.preprocess(sc.packed(), sc.count, u, fp64=True)["conic"][g]
+    assert c32[0] < 0 and c32[2] < 0 and c32[0] * c32[2] - c32[1] ** 2 < 0
+    assert c64[0] > 0 and c64[2] > 0 and c64[0] * c64[2] - c64[1] ** 2 > 0
+    a, _, _, _, _ = orc.render(sc.packed(), sc.count, u)
+    base, bcam = edge_frame()
+    b, _, _, _, _ = orc.render(base.packed(), base.count, bcam.uniform(base.count))
+    assert (np.abs(a - b).max(0) > 0).sum() > 0
+
+
 def test_edge_frame_fp32_vs_fp64(orc):
     """The whole edge frame (near-plane, behind-camera, off-screen, degenerate
     and huge splats, near needles): FP32 and FP64 frames take the same tile
