@@ -183,6 +183,7 @@ __device__ __forceinline__ SplatRec make_rec(const float4& conic, const float2& 
     r.co = conic;
     r.pq = make_float4(xy.x, xy.y, reach_qmax(conic.w), 0.0f);
     r.cl = rgb;
+    r.pad = make_float4(0.f, 0.f, 0.f, 0.f);
     return r;
 }
 
@@ -231,6 +232,7 @@ __global__ __launch_bounds__(256) void k_pack(const float4* __restrict__ conic, 
     const float2 p = xy[g];
     r.pq = make_float4(p.x, p.y, reach_qmax(r.co.w), 0.0f);
     r.cl = rgb[g];
+    r.pad = make_float4(0.f, 0.f, 0.f, 0.f);
     rec[g] = r;
 }
 
