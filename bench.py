@@ -34,11 +34,13 @@ FP32_PEAK_TFLOPS = 157.3
 # 14) plus the exponential, counted as one.
 RASTER_OPS_PER_EVALUATION = 15
 # HBM bytes per launch of the forward raster at this workload, from the
-# rocprofv3 FETCH_SIZE / WRITE_SIZE runs summarised in
-# profiles/forward_1m_1080p_pmc_traffic.md (calibrated against a plain copy);
-# a counter run cannot be part of this process, so the figure is the profile's.
-RASTER_TRAFFIC_BYTES = 622588548
-RASTER_TRAFFIC_SOURCE = "profiles/forward_1m_1080p_pmc_traffic.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)"
+# rocprofv3 request-size counters (TCC_EA0_RDREQ_32B/64B/128B,
+# TCC_EA0_WRREQ/_64B) summarised in profiles/forward_1m_1080p_pmc_traffic.md,
+# which FETCH_SIZE / WRITE_SIZE calibrated against a plain copy confirm; a
+# counter run cannot be part of this process, so the figure is the profile's.
+RASTER_TRAFFIC_BYTES = 459517723
+RASTER_TRAFFIC_SOURCE = ("profiles/forward_1m_1080p_pmc_traffic.md (rocprofv3 read/write request counters by size; "
+                         "FETCH_SIZE + WRITE_SIZE calibrated against a copy agree)")
 
 
 def log(msg):
