@@ -174,7 +174,7 @@ __device__ __forceinline__ void backward_block(uint32_t W, uint32_t H, uint32_t 
         gb = dLdC[2 * plane + o];
     }
     const uint32_t maxlast = wave_max_u32(lst);
-    if (maxlast == 0) return;
+    if (maxlast == 0 || splats == 0) return;  // nothing blended (no record to read)
     const uint32_t stop = min(range_end, start + maxlast);  // exclusive end of the replay
     if (stop <= start) return;
     const float rxa = (float)x0, rxb = (float)min(x0 + 7, W - 1);
@@ -187,22 +187,16 @@ __device__ __forceinline__ void backward_block(uint32_t W, uint32_t H, uint32_t 
     uint32_t n = 0;       // splats collected in the chunk buffer (wave-uniform)
     uint64_t slots = 0;   // their slots in the staged batch, 6 bits each
     for (int64_t b = (int64_t)start + (int64_t)((stop - start - 1) / 64) * 64; b >= (int64_t)start; b -= 64) {
+        // as in the forward: lanes without a valid instance read a valid
+        // record (the replay's last value, splat 0) and are not live
         const uint32_t idx = (uint32_t)b + lane;
-        float4 co = make_float4(0.f, 0.f, 0.f, 0.f), cl = make_float4(0.f, 0.f, 0.f, 0.f);
-        float2 p = make_float2(0.f, 0.f);
-        uint32_t gid = 0xFFFFFFFFu;
-        bool live = false;
-        if (idx < stop) {
-            const uint32_t g = vals[idx];
-            if (g < splats) {
-                gid = g;
-                const SplatRec sr = rec[g];
-                co = sr.co;
-                p = make_float2(sr.pq.x, sr.pq.y);
-                cl = sr.cl;
-                live = reaches_rect(p.x, p.y, co.x, co.y, co.z, sr.pq.z, rxa, rxb, rya, ryb);
-            }
-        }
+        const uint32_t g = vals[min(idx, stop - 1)];
+        const bool valid = idx < stop && g < splats;
+        const uint32_t gid = valid ? g : 0xFFFFFFFFu;
+        const SplatRec sr = rec[valid ? g : 0u];
+        const float4 co = sr.co, cl = sr.cl;
+        const float2 p = make_float2(sr.pq.x, sr.pq.y);
+        const bool live = valid && reaches_rect(p.x, p.y, co.x, co.y, co.z, sr.pq.z, rxa, rxb, rya, ryb);
         // Stage the batch in this wave's LDS slots (as in the forward).
         __builtin_amdgcn_wave_barrier();
         L.spl[lane].co = co;

@@ -108,22 +108,20 @@ __device__ __forceinline__ void raster_block(uint32_t W, uint32_t H, uint32_t x0
     uint32_t lst = 0;
     const float rxa = (float)x0, rxb = (float)min(x0 + 7, W - 1);
     const float rya = (float)y0, ryb = (float)min(y0 + 7, H - 1);
-    for (uint32_t b = start; b < end; b += 64) {
+    // Lanes past the range or holding a value that names no splat read a
+    // valid record anyway (the range's last value, splat 0) and are not
+    // live: no zero-filled registers and no branch around the loads.  With
+    // no splats there is no record to read and no instance can be valid.
+    const uint32_t bend = splats != 0 ? end : start;
+    for (uint32_t b = start; b < bend; b += 64) {
         if (ballot(!done) == 0) break;
         const uint32_t idx = b + lane;
-        float4 co = make_float4(0.f, 0.f, 0.f, 0.f), cl = make_float4(0.f, 0.f, 0.f, 0.f);
-        float2 p = make_float2(0.f, 0.f);
-        bool live = false;
-        if (idx < end) {
-            const uint32_t g = vals[idx];
-            if (g < splats) {
-                const SplatRec sr = rec[g];
-                co = sr.co;
-                p = make_float2(sr.pq.x, sr.pq.y);
-                cl = sr.cl;
-                live = reaches_rect(p.x, p.y, co.x, co.y, co.z, sr.pq.z, rxa, rxb, rya, ryb);
-            }
-        }
+        const uint32_t g = vals[min(idx, end - 1)];
+        const bool valid = idx < end && g < splats;
+        const SplatRec sr = rec[valid ? g : 0u];
+        const float4 co = sr.co, cl = sr.cl;
+        const float2 p = make_float2(sr.pq.x, sr.pq.y);
+        const bool live = valid && reaches_rect(p.x, p.y, co.x, co.y, co.z, sr.pq.z, rxa, rxb, rya, ryb);
         // Stage the batch in this wave's LDS slots.  Only this wave touches
         // them: a wave-level barrier orders the previous batch's reads before
         // these writes and these writes before the reads below.
