@@ -179,42 +179,72 @@ __device__ __forceinline__ void raster_block(uint32_t W, uint32_t H, uint32_t x0
 // here, tiles bucketed by instance count on a log scale (4 classes per
 // octave) and the classes in descending order, so the longest tiles start
 // first and the short ones fill in behind them instead of leaving a tail.
-// One workgroup per range of tiles (its LDS holds the 128 class counters):
-// with views (nullable; views[v].w = first tile of view v) workgroup v orders
-// view v's tiles within their own slots, so the order stays view by view and
-// a pass over it touches one view's per-splat data at a time.
+// One workgroup per range of tiles: with views (nullable; views[v].w = first
+// tile of view v) workgroup v orders view v's tiles within their own slots,
+// so the order stays view by view and a pass over it touches one view's
+// per-splat data at a time.
+// Each class has kOrderCopies counters, lane l of a wave using copy l % 32:
+// neighbouring tiles mostly share a class, and LDS atomics of a wave's lanes
+// on one address are serialised, which with a single counter per class took
+// most of this kernel's time.  The order within a class follows
+// (copy, arrival) and is any permutation of the class's tiles.
+constexpr uint32_t kOrderCopies = 32;
+constexpr uint32_t kOrderItems = 8;
 __global__ __launch_bounds__(1024) void k_tile_order(const uint2* __restrict__ ranges, uint32_t tiles,
                                                      const uint4* __restrict__ views, uint32_t num_views,
                                                      uint32_t* __restrict__ order) {
-    __shared__ uint32_t s_cnt[128];
-    const uint32_t tid = threadIdx.x;
+    constexpr uint32_t NC = 128 * kOrderCopies;  // counters, class-major (4 per thread)
+    __shared__ uint32_t s_cnt[NC];
+    __shared__ uint32_t s_wave[16];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     uint32_t t0 = 0, t1 = tiles;
     if (views) {
         t0 = views[blockIdx.x].w;
         t1 = blockIdx.x + 1 < num_views ? views[blockIdx.x + 1].w : tiles;
     }
-    if (tid < 128) s_cnt[tid] = 0;
+    for (uint32_t i = tid; i < NC; i += 1024) s_cnt[i] = 0;
     __syncthreads();
-    auto cls = [](uint32_t c) -> uint32_t {
-        if (c < 4) return c;
-        const uint32_t lg = 31 - __builtin_clz(c);
-        return min(127u, 4 * lg + ((c >> (lg - 2)) & 3u) - 4);
+    auto cls = [](uint32_t c) -> uint32_t {  // without a branch (selects only)
+        const uint32_t lg = 31 - __builtin_clz(max(c, 4u));
+        const uint32_t big = min(127u, 4 * lg + ((c >> (lg - 2)) & 3u) - 4);
+        return c < 4 ? c : big;
     };
-    for (uint32_t t = t0 + tid; t < t1; t += 1024) {
-        const uint2 r = ranges[t];
-        atomicAdd(&s_cnt[127 - cls(r.y - r.x)], 1u);
+    // counter of tile t for this lane: heaviest class first, then the copy
+    auto slot_of = [&](uint2 r) -> uint32_t { return (127 - cls(r.y - r.x)) * kOrderCopies + (lane % kOrderCopies); };
+    // Each pass loads kOrderItems ranges per thread before using any (one
+    // memory round trip per 8192 tiles).
+    for (uint32_t c0 = t0; c0 < t1; c0 += 1024 * kOrderItems) {
+        uint2 r[kOrderItems];
+#pragma unroll
+        for (uint32_t i = 0; i < kOrderItems; ++i) r[i] = ranges[min(c0 + i * 1024 + tid, t1 - 1)];  // in range
+#pragma unroll
+        for (uint32_t i = 0; i < kOrderItems; ++i)
+            if (c0 + i * 1024 + tid < t1) atomicAdd(&s_cnt[slot_of(r[i])], 1u);
     }
     __syncthreads();
-    if (tid < 64) {
-        const uint32_t a = s_cnt[2 * tid], b = s_cnt[2 * tid + 1];
-        const uint32_t incl = wave_inclusive_sum(a + b);
-        s_cnt[2 * tid] = incl - a - b;
-        s_cnt[2 * tid + 1] = incl - b;
+    {   // exclusive scan of the counters in index order: 4 per thread
+        const uint32_t c[4] = {s_cnt[4 * tid], s_cnt[4 * tid + 1], s_cnt[4 * tid + 2], s_cnt[4 * tid + 3]};
+        const uint32_t sum = c[0] + c[1] + c[2] + c[3];
+        const uint32_t incl = wave_inclusive_sum(sum);
+        if (lane == 63) s_wave[wave] = incl;
+        __syncthreads();
+        uint32_t base = incl - sum;
+        for (uint32_t w = 0; w < wave; ++w) base += s_wave[w];
+        s_cnt[4 * tid] = base;
+        s_cnt[4 * tid + 1] = base + c[0];
+        s_cnt[4 * tid + 2] = base + c[0] + c[1];
+        s_cnt[4 * tid + 3] = base + c[0] + c[1] + c[2];
     }
     __syncthreads();
-    for (uint32_t t = t0 + tid; t < t1; t += 1024) {
-        const uint2 r = ranges[t];
-        order[t0 + atomicAdd(&s_cnt[127 - cls(r.y - r.x)], 1u)] = t;
+    for (uint32_t c0 = t0; c0 < t1; c0 += 1024 * kOrderItems) {
+        uint2 r[kOrderItems];
+#pragma unroll
+        for (uint32_t i = 0; i < kOrderItems; ++i) r[i] = ranges[min(c0 + i * 1024 + tid, t1 - 1)];  // in range
+#pragma unroll
+        for (uint32_t i = 0; i < kOrderItems; ++i) {
+            const uint32_t t = c0 + i * 1024 + tid;
+            if (t < t1) order[t0 + atomicAdd(&s_cnt[slot_of(r[i])], 1u)] = t;
+        }
     }
 }
 
