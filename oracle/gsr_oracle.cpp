@@ -26,7 +26,8 @@
 // formulas) measure the FP32 error.
 //
 // Pins (tests/test_oracle.py): exp_f32 against libm within 2 ulp, exp_blend_f32
-// within 2.7 ulp on [-5.55, 0]; the FP32 blend decision (blend_range_f32 and
+// within 2.7 ulp on [-5.55, 0] and its ldexp equal, on every float there, to
+// the kernels' exponent insertion by shift-and-add; the FP32 blend decision (blend_range_f32 and
 // the alpha tests) against FP64 libm over every power from 0 down to -inf,
 // and a raster of far thin splats against the FP64 raster; the integer
 // chain against closed forms / brute force; preprocess special cases
@@ -88,20 +89,25 @@ inline float exp_f32(float x) {
 // (coefficients fitted by iteratively reweighted least squares for relative
 // error on [-1/2, 1/2], oracle/exp_blend_fit.py), times 2^k.  Pinned within
 // 2.7 ulp of libm's exp on [-5.55, 0] (tests/test_oracle.py).
-inline float exp_blend_f32(float p) {
+// The two parts of exp_blend_f32: q = 2^f and the bits of t = 1.5 * 2^23 + k.
+inline void exp_blend_parts(float p, float& q, uint32_t& tu) {
     const float kLog2e = 0x1.715476p+0f, kRound = 0x1.8p+23f;
     const float t = fmaf(p, kLog2e, kRound);
     const float f = fmaf(p, kLog2e, -(t - kRound));
-    float q = 0x1.5bba1p-10f;
+    q = 0x1.5bba1p-10f;
     q = fmaf(q, f, 0x1.3cea88p-7f);
     q = fmaf(q, f, 0x1.c6b752p-5f);
     q = fmaf(q, f, 0x1.ebf9bcp-3f);
     q = fmaf(q, f, 0x1.62e42ap-1f);
     q = fmaf(q, f, 1.0f);
-    uint32_t tu, ru;
     std::memcpy(&tu, &t, 4);
-    std::memcpy(&ru, &kRound, 4);
-    return ldexpf(q, (int32_t)(tu - ru));
+}
+
+inline float exp_blend_f32(float p) {
+    float q;
+    uint32_t tu;
+    exp_blend_parts(p, q, tu);
+    return ldexpf(q, (int32_t)(tu - 0x4B400000u));  // 0x4B400000: bits of 1.5 * 2^23
 }
 
 // Which powers the FP32 blend evaluates.  raseter.wgsl:84-92 skips a splat
@@ -1167,6 +1173,29 @@ float orc_exp_f32(float x) { return exp_f32(x); }
 
 void orc_exp_blend_f32_array(const float* x, float* y, int64_t n) {
     for (int64_t i = 0; i < n; ++i) y[i] = exp_blend_f32(x[i]);
+}
+
+// The GPU kernels scale 2^f by 2^k by adding bits(t) << 23 to q's bits (the
+// bits of 1.5 * 2^23 vanish in that shift) instead of ldexp.  Counts the
+// floats p = -bits_as_float(b), b in [b_lo, b_hi], where that differs from
+// exp_blend_f32's ldexp in any bit (tests: none on [-5.55, 0]; some where
+// the result underflows, which shows the comparison sees a difference).
+int64_t orc_exp_blend_shift_mismatches(uint32_t b_lo, uint32_t b_hi) {
+    int64_t bad = 0;
+#pragma omp parallel for reduction(+ : bad) schedule(static)
+    for (int64_t b = b_lo; b <= (int64_t)b_hi; ++b) {
+        const uint32_t pb = 0x80000000u | (uint32_t)b;
+        float p, q;
+        uint32_t tu;
+        std::memcpy(&p, &pb, 4);
+        exp_blend_parts(p, q, tu);
+        const float a = ldexpf(q, (int32_t)(tu - 0x4B400000u));
+        uint32_t qb, ab;
+        std::memcpy(&qb, &q, 4);
+        std::memcpy(&ab, &a, 4);
+        bad += (ab != qb + (tu << 23)) ? 1 : 0;
+    }
+    return bad;
 }
 
 // The FP32 blend's alpha for (power, opacity) pairs, 0 where the pixel skips

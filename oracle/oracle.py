@@ -36,6 +36,8 @@ def lib():
         L.orc_exp_f32_array.argtypes = [_f32p, _f32p, ctypes.c_int64]
         L.orc_exp_blend_f32_array.argtypes = [_f32p, _f32p, ctypes.c_int64]
         L.orc_blend_alpha_f32.argtypes = [_f32p, _f32p, _f32p, ctypes.c_int64]
+        L.orc_exp_blend_shift_mismatches.restype = ctypes.c_int64
+        L.orc_exp_blend_shift_mismatches.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
         L.orc_num_threads.restype = ctypes.c_int
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_preprocess_f32.argtypes = [_f32p, ctypes.c_uint32, _f32p, ctypes.c_uint32,
@@ -96,6 +98,15 @@ def blend_alpha_f32(power, op):
     a = np.empty_like(p)
     lib().orc_blend_alpha_f32(p.ravel(), o.ravel(), a.ravel(), p.size)
     return a
+
+
+def exp_blend_shift_mismatches(mag_lo, mag_hi):
+    """How many powers p in [-mag_hi, -mag_lo] (every float) give a different
+    exp_blend result when 2^f is scaled by adding bits(t) << 23 to its bits
+    (the GPU kernels' form) instead of by ldexp."""
+    lo = int(np.float32(mag_lo).view(np.uint32))
+    hi = int(np.float32(mag_hi).view(np.uint32))
+    return int(lib().orc_exp_blend_shift_mismatches(lo, hi))
 
 
 def preprocess(packed, n, uni, tile=16, fp64=False):

@@ -56,6 +56,16 @@ def _blend_alpha_f64(power, op):
     return np.where((p > 0) | (a < np.float64(np.float32(1.0 / 255.0))), 0.0, a)
 
 
+def test_exp_blend_exponent_by_shift_equals_ldexp(orc):
+    """The GPU kernels scale the blend exponential's 2^f by 2^k with one
+    shift-and-add on the bits; the oracle uses ldexp.  Every float power in
+    [-5.55, 0] gives the same bits both ways (exhaustive: 1.08e9 floats).
+    Below about -87 the result underflows, where the bit form is wrong: the
+    check sees that."""
+    assert orc.exp_blend_shift_mismatches(0.0, 5.55) == 0
+    assert orc.exp_blend_shift_mismatches(100.0, 104.0) > 0
+
+
 def test_blend_decision_matches_fp64_at_every_power(orc):
     """The FP32 blend (power range cut, exp_blend_f32, 0.99 clamp, 1/255 test) takes
     the FP64 decision at every power from 0 down to -inf, including the powers

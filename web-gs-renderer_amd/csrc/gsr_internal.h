@@ -51,7 +51,12 @@ __device__ __forceinline__ float exp_core(float x) {
 // fraction f = p log2 e - k (|f| <= 1/2) from one fused multiply-add, 2^f by a
 // degree-5 polynomial fitted for relative error on [-1/2, 1/2], scaled by 2^k.
 // Within 2.7 ulp of e^p for every float in [-5.55, 0].  The oracle's
-// exp_blend_f32 is the same sequence.
+// exp_blend_f32 is the same sequence.  The scaling adds k to the exponent
+// field in one instruction: t's bits are those of 1.5 * 2^23 plus k, and
+// 1.5 * 2^23's bits shifted left by 23 wrap to 0, so (bits(t) << 23) is
+// k << 23.  On [-5.55, 0] (k in [-9, 0], 2^f in [0.7, 1.42]) the result is a
+// normal float and equals ldexp(2^f, k) bit for bit; elsewhere it may not,
+// and those lanes' values are never used (a NaN power stays NaN).
 __device__ __forceinline__ float exp_blend(float p) {
     constexpr float kLog2e = 0x1.715476p+0f, kRound = 0x1.8p+23f;
     const float t = __builtin_fmaf(p, kLog2e, kRound);
@@ -62,7 +67,8 @@ __device__ __forceinline__ float exp_blend(float p) {
     q = __builtin_fmaf(q, f, 0x1.ebf9bcp-3f);
     q = __builtin_fmaf(q, f, 0x1.62e42ap-1f);
     q = __builtin_fmaf(q, f, 1.0f);
-    return __builtin_ldexpf(q, (int)(__float_as_uint(t) - __float_as_uint(kRound)));
+    static_assert((0x4B400000u << 23) == 0u, "the rounding constant's bits vanish in the shift");
+    return __uint_as_float(__float_as_uint(q) + (__float_as_uint(t) << 23));
 }
 
 __device__ __forceinline__ float gsr_exp(float x) {
