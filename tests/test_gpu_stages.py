@@ -276,3 +276,26 @@ def test_raster_bit_exact(torch_cuda, ctx, orc, tile):
     assert np.array_equal(img, img_o)
     assert np.array_equal(to_host(T, np.float32).reshape(H, W), T_o)
     assert np.array_equal(to_host(last, np.uint32).reshape(H, W), last_o)
+
+
+def test_raster_without_splats_is_black(torch_cuda, ctx):
+    """gsr_rasterize with no splats but ranges that name instances (values
+    that cannot name a splat): every pixel keeps T = 1 and no contributor,
+    and no record is read."""
+    W, H, tile = 40, 24, 16
+    TX, TY = (W + tile - 1) // tile, (H + tile - 1) // tile
+    vs = np.array([3, 9, 1, 7], np.uint32)
+    ranges = np.zeros((TX * TY, 2), np.uint32)
+    ranges[0] = (0, 4)
+    ranges[TX * TY - 1] = (2, 4)
+    empty = to_dev(torch_cuda, np.zeros(4, np.float32))
+    out = empty_dev(torch_cuda, 3 * W * H, np.float32)
+    T = empty_dev(torch_cuda, W * H, np.float32)
+    last = empty_dev(torch_cuda, W * H, np.uint32)
+    d_r, d_v = to_dev(torch_cuda, ranges.ravel()), to_dev(torch_cuda, vs)
+    torch_cuda.cuda.synchronize()
+    _say("raster without splats")
+    ctx.rasterize(W, H, tile, 0, vs.size, d_r, d_v, empty, empty, empty, out, T, last)
+    assert not to_host(out, np.float32).any()
+    assert (to_host(T, np.float32) == 1.0).all()
+    assert not to_host(last, np.uint32).any()
