@@ -299,3 +299,33 @@ def test_raster_without_splats_is_black(torch_cuda, ctx):
     assert not to_host(out, np.float32).any()
     assert (to_host(T, np.float32) == 1.0).all()
     assert not to_host(last, np.uint32).any()
+
+
+@pytest.mark.parametrize("n,key_bytes,bits", [((1 << 30) - 1, 4, 29), (300_000_001, 8, 34)])
+def test_sort_at_the_key_limit(torch_cuda, ctx, n, key_bytes, bits):
+    """The sort at its documented limit (2^30 - 1 keys: the look-back words
+    carry 30-bit counts) and 300M 64-bit keys, generated and checked on the
+    GPU: keys and values equal torch's stable sort (values are the input
+    positions, so ties must keep input order), and one key more is refused."""
+    torch = torch_cuda
+    _say(f"sort at the limit n={n} key_bytes={key_bytes} bits={bits}")
+    g = torch.Generator(device="cuda").manual_seed(n % 1000)
+    kt = torch.int32 if key_bytes == 4 else torch.int64
+    keys = torch.randint(0, 1 << bits, (n,), dtype=kt, device="cuda", generator=g)
+    keys[: n // 4] = keys[: n // 4] & 0xFF  # many ties
+    k1 = torch.empty_like(keys)
+    v0 = torch.arange(n, dtype=torch.int32, device="cuda")
+    v1 = torch.empty_like(v0)
+    k0 = keys.clone()
+    torch.cuda.synchronize()
+    idx = ctx.sort_pairs(key_bytes, k0, v0, k1, v1, n, 0, bits)
+    ks, vs = (k1, v1) if idx else (k0, v0)
+    ref_k, ref_i = torch.sort(keys, stable=True)
+    assert torch.equal(ks, ref_k)
+    assert torch.equal(vs, ref_i.to(torch.int32))
+    del ref_k, ref_i, ks, vs, k0, k1, v0, v1, keys
+    torch.cuda.empty_cache()
+    if key_bytes == 4:  # refused before any buffer is touched
+        d = torch.empty(1, dtype=torch.int32, device="cuda")
+        with pytest.raises(lib.GsrError, match="exceed the sort limit"):
+            ctx.sort_pairs(4, d, d, d, d, 1 << 30, 0, 32)
