@@ -186,3 +186,21 @@ def test_empty_scene_renders_black(torch_cuda, ctx):
     ctx.synchronize()
     assert not bout.any().item() and ctx.batch_instance_count() == 0
     ctx.scene_free("empty")
+
+
+def test_render_4k_8px_tiles_17bit_tile_ids(torch_cuda, ctx, orc):
+    """3840x2160 at 8-pixel tiles: 129,600 tiles, so tile ids need 17 bits and
+    the frame sorts 64-bit (tile << 16 | depth16) keys on its own (33 bits).
+    Image bit-exact and instance count exact against the FP32 oracle, which
+    builds the same 64-bit keys (emit.wgsl:62)."""
+    sc = scenes.clustered_scene(200_000, seed=4)
+    cam = scenes.Camera((0.0, 0.0, -8.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 50.0, 3840, 2160)
+    u = cam.uniform(sc.count)
+    _say("render 200k splats 3840x2160 tile 8")
+    ctx.scene_upload("k4", sc.packed(), sc.count)
+    ctx.scene_select("k4")
+    img = ctx.render(u, lib.options(tile_size=8))
+    ref, _, _, I, _ = orc.render(sc.packed(), sc.count, u, tile=8)
+    assert ctx.instance_count() == I
+    assert np.array_equal(img, ref), np.abs(img - ref).max()
+    ctx.scene_free("k4")
