@@ -360,6 +360,21 @@ struct ParRegs {
     const float* v;  // the 59 channels
     __device__ float operator()(int ch) const { return v[ch]; }
 };
+// The 48 SH channels from a thread's column of an LDS table (row k: SH value
+// k, f_dc then f_rest), the others from the planar scene.
+constexpr int kShChannels = 48;
+struct ParShLds {
+    const float* __restrict__ P;
+    size_t N;
+    uint32_t g;
+    const float (*sh)[256];
+    uint32_t col;
+    __device__ float operator()(int ch) const {
+        if (ch >= CH_DC && ch < CH_DC + 3) return sh[ch - CH_DC][col];
+        if (ch >= CH_REST) return sh[3 + ch - CH_REST][col];
+        return P[(size_t)ch * N + g];
+    }
+};
 
 template <typename Par>
 __device__ __forceinline__ void splat_shape(const Par& par, SplatShape& s) {
@@ -670,9 +685,19 @@ __global__ __launch_bounds__(256) void k_preprocess_backward_batch(const float* 
                                                                    const float4* __restrict__ grad2d,
                                                                    float4* __restrict__ grad2d_out,
                                                                    float* __restrict__ grad) {
+    // The SH coefficients are read per view; from HBM that was 192 B per
+    // touched (view, splat), which missed L2 (the per-view sums stream past
+    // it).  Each thread keeps its own in a column of this table (48 KB per
+    // workgroup, which the kernel's 3 waves per SIMD leave room for); only
+    // the thread itself reads its column, so no barrier is needed.
+    __shared__ float s_sh[kShChannels][256];
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
     const size_t N = n;
+    const uint32_t col = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < kShChannels; ++k)
+        s_sh[k][col] = __builtin_nontemporal_load(P + (size_t)(k < 3 ? CH_DC + k : CH_REST + k - 3) * N + g);
     SplatCov cv;
     bool shaped = false, touched = false;
     float sp[3] = {0.f, 0.f, 0.f}, ss[6] = {}, sop = 0.0f, sch[3] = {0.f, 0.f, 0.f}, srest[45] = {};
@@ -689,7 +714,7 @@ __global__ __launch_bounds__(256) void k_preprocess_backward_batch(const float* 
             grad2d_out[e * 3 + 2] = make_float4(Gv[8], 0.0f, 0.0f, 0.0f);
         }
         if (!any) continue;
-        const float* Pv = opaque(P);  // SH coefficients re-read per view
+        const float* Pv = opaque(P);  // the shape read where it is needed, not held through the loop
         if (!shaped) {
             SplatShape full;
             splat_shape(ParGlobal{Pv, N, g}, full);
@@ -703,7 +728,7 @@ __global__ __launch_bounds__(256) void k_preprocess_backward_batch(const float* 
             shaped = true;
         }
         float gp[3], gsym[3][3], gch[3], basis[15];
-        if (!view_adjoint(ParGlobal{Pv, N, g}, cv, cams[v], Gv, gp, gsym, gch, basis)) continue;
+        if (!view_adjoint(ParShLds{Pv, N, g, s_sh, col}, cv, cams[v], Gv, gp, gsym, gch, basis)) continue;
         touched = true;
 #pragma unroll
         for (int c = 0; c < 3; ++c) sp[c] += gp[c];
