@@ -66,7 +66,7 @@ typedef struct gsr_options {
     uint32_t timing;           /* 1 = hipEvents around every stage (gsr_get_timings);            */
                                /* 2 = around the raster stages only (two events per raster)     */
     uint32_t save_state;       /* 1 = keep per-pixel T_final / last contributor for backward     */
-    uint32_t raster_waves_per_cu; /* batches: persistent raster waves per CU; 0 = 16              */
+    uint32_t raster_waves_per_cu; /* batches: persistent raster waves per CU; 0 = 32              */
     uint32_t sort_blocks_per_cu;  /* onesweep blocks per CU: 1, 2 or 4 (1024-, 512-, 256-thread); 0 = 1 */
     uint32_t sort_digit_bits;     /* radix digit width: 8 to 11 bits (256 to 2048 buckets); 0 = 8 for
                                      32-bit keys, for 64-bit keys the narrowest of 8..10 with the

@@ -389,7 +389,7 @@ int launch_raster_queue(Context* c, uint32_t tile, const uint4* views, const uin
         return fail(c, GSR_ERR_ARGUMENT, "batch raster: T_final and last contributor go together");
     const uint32_t tickets = ceil_div(total_tiles * (tile * tile / 64), 4u);
     const uint32_t groups =
-        std::max<uint32_t>(1, std::min<uint32_t>(tickets, (uint32_t)c->cu_count * (waves_per_cu ? waves_per_cu : 16) / 4));
+        std::max<uint32_t>(1, std::min<uint32_t>(tickets, (uint32_t)c->cu_count * (waves_per_cu ? waves_per_cu : 32) / 4));
     const uint2* rg = (const uint2*)ranges;
     switch (tile) {
         case 8:
