@@ -270,7 +270,8 @@ int launch_emit(Context* c, const uint32_t* offsets, const uint4* meta, uint32_t
 // range and digit width), so the sort skips its own clearing launch.
 // mem (nullable): caller-provided scratch for the histograms and look-back
 // words instead of the context's buffers (no allocation); it must hold
-// sort_scratch_words() of each kind.
+// sort_scratch_words() of each kind (one histogram copy; the context's
+// buffers hold several, see sort.hip).
 struct SortMem {
     uint32_t* hist;
     size_t hist_words;
@@ -284,8 +285,9 @@ int launch_sort(Context* c, uint32_t key_bytes, void* k0, uint32_t* v0, void* k1
                 const uint32_t* n_dev, uint64_t n_host, uint64_t capacity, uint32_t begin_bit,
                 uint32_t end_bit, uint32_t blocks_per_cu, FrameCounters* counters, int* result_index,
                 uint32_t digit_bits = 8, bool hist_cleared = false, const SortMem* mem = nullptr);
-// Scratch words launch_sort needs for `capacity` keys with 1 block per CU and
-// 8-bit digits (other geometries: SIZE_MAX).
+// Scratch words launch_sort needs in caller-provided memory (mem) for
+// `capacity` keys with 1 block per CU and 8-bit digits (other geometries:
+// SIZE_MAX).
 SortWords sort_scratch_words(const Context* c, uint32_t key_bytes, uint64_t capacity, uint32_t begin_bit,
                              uint32_t end_bit, uint32_t blocks_per_cu, uint32_t digit_bits);
 // Keys per full onesweep tile of that geometry.

@@ -10,7 +10,8 @@
 //                    tiles the device-side count needs); one read of the
 //                    keys builds every pass's digit histogram (LDS bins; a
 //                    run of equal digits in consecutive lanes adds once; one
-//                    global atomic per bin and block, one block per CU);
+//                    global atomic per bin and block, into one of several
+//                    copies of the histograms that onesweep sums);
 //   k_onesweep x P   persistent blocks take tiles by ticket, rank keys per
 //                    wave64 with ballot digit matching (stable), chain
 //                    per-digit tile prefixes with decoupled look-back, stage
@@ -44,6 +45,12 @@ namespace {
 constexpr uint32_t kAgg = 1u << 30, kInc = 2u << 30;
 constexpr int kLookBack = 16;   // predecessor words fetched per look-back step (all digits of a thread)
 constexpr int kHistItems = 8;   // keys per thread per histogram block tile
+// Copies of the global digit histograms that the histogram blocks add into
+// (block b into copy b % kHistCopies; onesweep sums them): one copy took the
+// flush of some 400K atomics into 32 cache lines (1080p frame), which
+// serialise per line; 8 copies: frame sort 0.1148 -> 0.1122 ms.  Scratch the
+// caller provides (the no-allocation sort) keeps one copy.
+constexpr uint32_t kHistCopies = 8;
 // onesweep keys per thread at most for 64-bit keys in 1024-thread blocks
 // (8 in smaller blocks, whose several blocks per CU need fewer registers)
 constexpr int kMaxItems64 = 12;
@@ -163,7 +170,7 @@ __global__ __launch_bounds__(1024) void k_sort_hist(const K* __restrict__ keys, 
                                                    uint64_t n_host, uint32_t begin_bit, uint32_t end_bit,
                                                    uint32_t passes, uint32_t* __restrict__ hist,
                                                    uint32_t* __restrict__ status0, uint32_t slots,
-                                                   uint32_t sweep_threads, uint32_t max_items) {
+                                                   uint32_t sweep_threads, uint32_t max_items, uint32_t copies) {
     constexpr int R = 1 << RB;
     constexpr int COPIES = RB <= 8 ? 4 : 1;
     constexpr int MAXP = (int)(sizeof(K) * 8 + RB - 1) / RB;
@@ -214,7 +221,7 @@ __global__ __launch_bounds__(1024) void k_sort_hist(const K* __restrict__ keys, 
         uint32_t v = 0;
 #pragma unroll
         for (int w = 0; w < COPIES; ++w) v += s_hist[w][i];
-        if (v) atomicAdd(&hist[i], v);
+        if (v) atomicAdd(&hist[(size_t)(blockIdx.x % copies) * passes * R + i], v);
     }
 }
 
@@ -456,7 +463,8 @@ template <typename K, int THREADS, int RB>
 __global__ __launch_bounds__(THREADS) void k_onesweep(
     const K* __restrict__ kin, const uint32_t* __restrict__ vin, K* __restrict__ kout,
     uint32_t* __restrict__ vout, const uint32_t* n_dev, uint64_t n_host, uint32_t shift, uint32_t bits,
-    const uint32_t* __restrict__ hist, uint32_t* ticket, uint32_t* status, uint32_t* clear_next,
+    const uint32_t* __restrict__ hist, uint32_t hist_copies, uint32_t hist_stride, uint32_t* ticket,
+    uint32_t* status, uint32_t* clear_next,
     uint32_t* error, uint32_t slots) {
     using G = SortGeom<K, THREADS, RB>;
     constexpr int ITEMS = G::kItems, R = G::kRadixN, DPT = G::kDPT;
@@ -480,7 +488,12 @@ __global__ __launch_bounds__(THREADS) void k_onesweep(
     {
         uint32_t h[DPT], g[DPT];
 #pragma unroll
-        for (int k = 0; k < DPT; ++k) h[k] = digit_thread ? hist[d0 + k] : 0u;
+        for (int k = 0; k < DPT; ++k) {
+            uint32_t s = 0;
+            if (digit_thread)
+                for (uint32_t cp = 0; cp < hist_copies; ++cp) s += hist[(size_t)cp * hist_stride + d0 + k];
+            h[k] = s;
+        }
         digits_exclusive<G::kDigitWaves, DPT>(h, g, L.s_w);
         if (digit_thread)
 #pragma unroll
@@ -502,16 +515,18 @@ __global__ __launch_bounds__(THREADS) void k_onesweep(
         onesweep_tiles<K, THREADS, RB, I4>(L, kin, vin, kout, vout, n, num_tiles, shift, mask, ticket, status, error);
 }
 
-// Words of sort scratch for a geometry: the histograms and tickets, and two
-// sets of look-back words.  A pass has at most max(resident blocks, full
-// tiles of the capacity) tiles: the tile is shortened to give each resident
-// block one tile while the keys do not fill them all (onesweep_items).
+// Words of sort scratch for a geometry: the histograms (copies of them) and
+// tickets, and two sets of look-back words.  A pass has at most max(resident
+// blocks, full tiles of the capacity) tiles: the tile is shortened to give
+// each resident block one tile while the keys do not fill them all
+// (onesweep_items).
 template <typename K, int THREADS, int RB>
-SortWords sort_words_of(const Context* c, uint64_t capacity, uint32_t begin_bit, uint32_t end_bit) {
+SortWords sort_words_of(const Context* c, uint64_t capacity, uint32_t begin_bit, uint32_t end_bit,
+                        uint32_t copies) {
     using G = SortGeom<K, THREADS, RB>;
     const uint32_t slots = (uint32_t)c->cu_count * (1024 / THREADS);
     const uint64_t max_tiles = std::max<uint64_t>(slots, (capacity + G::kTile - 1) / G::kTile);
-    return SortWords{(size_t)sort_passes(begin_bit, end_bit, RB) * G::kRadixN + kMaxPasses,
+    return SortWords{(size_t)sort_passes(begin_bit, end_bit, RB) * G::kRadixN * copies + kMaxPasses,
                      (size_t)2 * max_tiles * G::kRadixN};
 }
 
@@ -526,7 +541,8 @@ int sort_impl(Context* c, K* k0, uint32_t* v0, K* k1, uint32_t* v1, const uint32
     if (passes == 0) return GSR_OK;
     if (passes > (uint32_t)kMaxPasses) return fail(c, GSR_ERR_ARGUMENT, "too many radix passes (%u)", passes);
     const uint32_t slots = (uint32_t)c->cu_count * (1024 / THREADS);
-    const SortWords words = sort_words_of<K, THREADS, RB>(c, capacity, begin_bit, end_bit);
+    const uint32_t copies = mem ? 1u : kHistCopies;
+    const SortWords words = sort_words_of<K, THREADS, RB>(c, capacity, begin_bit, end_bit, copies);
     const size_t hist_words = words.hist;
     const uint32_t max_tiles = (uint32_t)(words.status / (2 * R));
     uint32_t* hist;
@@ -542,7 +558,7 @@ int sort_impl(Context* c, K* k0, uint32_t* v0, K* k1, uint32_t* v1, const uint32
         status = (uint32_t*)scratch(c, "sort_status", words.status * 4);
     }
     if (!hist || !status) return GSR_ERR_ALLOC;
-    uint32_t* tickets = hist + (size_t)passes * R;
+    uint32_t* tickets = hist + (size_t)passes * R * copies;
     if (!hist_cleared)
         hipLaunchKernelGGL(k_sort_init, dim3((uint32_t)std::min<size_t>((hist_words + 255) / 256, 64)), dim3(256), 0,
                            c->stream, hist, hist_words);
@@ -552,7 +568,8 @@ int sort_impl(Context* c, K* k0, uint32_t* v0, K* k1, uint32_t* v1, const uint32
     const uint32_t hist_blocks = std::min<uint32_t>(
         std::max<uint32_t>(1, ceil_div(capacity, (uint64_t)1024 * kHistItems)), (uint32_t)c->cu_count * 4);
     hipLaunchKernelGGL((k_sort_hist<K, RB>), dim3(hist_blocks), dim3(1024), 0, c->stream, k0, n_dev, n_host,
-                       begin_bit, end_bit, passes, hist, status, slots, (uint32_t)THREADS, (uint32_t)G::kItems);
+                       begin_bit, end_bit, passes, hist, status, slots, (uint32_t)THREADS, (uint32_t)G::kItems,
+                       copies);
     K* kb[2] = {k0, k1};
     uint32_t* vb[2] = {v0, v1};
     uint32_t* st[2] = {status, status + (size_t)max_tiles * R};
@@ -561,8 +578,8 @@ int sort_impl(Context* c, K* k0, uint32_t* v0, K* k1, uint32_t* v1, const uint32
         const uint32_t bits = std::min<uint32_t>(RB, end_bit - shift);
         hipLaunchKernelGGL((k_onesweep<K, THREADS, RB>), dim3(slots), dim3(THREADS), 0, c->stream, kb[p & 1],
                            vb[p & 1], kb[(p + 1) & 1], vb[(p + 1) & 1], n_dev, n_host, shift, bits, hist + p * R,
-                           tickets + p, st[p & 1], p + 1 < passes ? st[(p + 1) & 1] : nullptr, &counters->error,
-                           slots);
+                           copies, passes * R, tickets + p, st[p & 1], p + 1 < passes ? st[(p + 1) & 1] : nullptr,
+                           &counters->error, slots);
     }
     GSR_HIP(c, hipGetLastError());
     return GSR_OK;
@@ -627,7 +644,8 @@ uint32_t sort_result_index(uint32_t begin_bit, uint32_t end_bit, uint32_t digit_
 }
 
 FillRegion sort_hist_region(Context* c, uint32_t begin_bit, uint32_t end_bit, uint32_t digit_bits) {
-    const size_t words = (size_t)sort_passes(begin_bit, end_bit, digit_bits) * (1u << digit_bits) + kMaxPasses;
+    const size_t words =
+        (size_t)sort_passes(begin_bit, end_bit, digit_bits) * (1u << digit_bits) * kHistCopies + kMaxPasses;
     uint32_t* p = (uint32_t*)scratch(c, "sort_hist", words * 4);
     return FillRegion{p, p ? words : 0};
 }
@@ -639,8 +657,8 @@ uint32_t sort_default_tile_keys(uint32_t key_bytes) {
 SortWords sort_scratch_words(const Context* c, uint32_t key_bytes, uint64_t capacity, uint32_t begin_bit,
                              uint32_t end_bit, uint32_t blocks_per_cu, uint32_t digit_bits) {
     if (digit_bits == 8 && blocks_per_cu == 1)
-        return key_bytes == 8 ? sort_words_of<uint64_t, 1024, 8>(c, capacity, begin_bit, end_bit)
-                              : sort_words_of<uint32_t, 1024, 8>(c, capacity, begin_bit, end_bit);
+        return key_bytes == 8 ? sort_words_of<uint64_t, 1024, 8>(c, capacity, begin_bit, end_bit, 1)
+                              : sort_words_of<uint32_t, 1024, 8>(c, capacity, begin_bit, end_bit, 1);
     return SortWords{~(size_t)0, ~(size_t)0};  // only the default geometry is sized here
 }
 
