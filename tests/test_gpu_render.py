@@ -204,3 +204,37 @@ def test_render_4k_8px_tiles_17bit_tile_ids(torch_cuda, ctx, orc):
     assert ctx.instance_count() == I
     assert np.array_equal(img, ref), np.abs(img - ref).max()
     ctx.scene_free("k4")
+
+
+def _tile_loads(orc, sc, u, tile=16):
+    """Instances per tile of the oracle's frame (preprocess -> scan -> emit -> sort -> ranges)."""
+    W, H = int(u[40]), int(u[41])
+    tx, ty = (W + tile - 1) // tile, (H + tile - 1) // tile
+    pre = orc.preprocess(sc.packed(), sc.count, u, tile)
+    offs, total = orc.scan(pre["tiles"])
+    offsets = np.append(offs, np.uint32(total))
+    keys, vals = orc.emit(offsets, pre["meta"], sc.count, W, tile, total)
+    keys, vals = orc.sort_stable(keys, vals)
+    r = orc.tile_ranges(keys, tx * ty).reshape(-1, 2)
+    return (r[:, 1] - r[:, 0]).astype(np.int64)
+
+
+def test_render_deep_tiles_matches_oracle(torch_cuda, ctx, orc):
+    """24,000 splats around the centre of a 160x128 frame: 4 central tiles
+    hold more than 16384 instances each (up to 23,997), 32 between 1025 and
+    16384, the rest at most 1024.  The image, and the instance count, must
+    equal the oracle's frame."""
+    sc = scenes.uniform_scene(24000, seed=5, extent=0.15, scale_mu=-2.5, scale_sigma=0.8)
+    cam = scenes.Camera((0.0, 0.0, -4.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 60.0, 160, 128)
+    u = cam.uniform(sc.count)
+    loads = _tile_loads(orc, sc, u)
+    _say(f"deep tiles: max {loads.max()}, tiles above 16384: {(loads > 16384).sum()}, "
+         f"in (1024, 16384]: {((loads > 1024) & (loads <= 16384)).sum()}, at most 1024: {(loads <= 1024).sum()}")
+    assert loads.max() > 16384
+    assert ((loads > 1024) & (loads <= 16384)).any() and ((loads > 1) & (loads <= 1024)).any()
+    ctx.scene_upload("deep", sc.packed(), sc.count)
+    ctx.scene_select("deep")
+    img = ctx.render(u)
+    ref, _, _, I, _ = orc.render(sc.packed(), sc.count, u)
+    assert ctx.instance_count() == I
+    assert np.array_equal(img, ref)
