@@ -65,14 +65,23 @@ struct BwdWaveLds {
 };
 static_assert(sizeof(BwdWaveLds) == 6144, "backward LDS per wave");
 
+// Which of the nine per-splat totals lane l16 = lane % 16 of a splat's
+// 16-lane group holds after backward_flush's reduce-scatter (4 bits per
+// lane, 0xF: a duplicate, not added).
+constexpr uint64_t kFlushTotalOfLane = 0xFFF8F765FF43F210ull;
+
 // Sums the collected splats' terms per splat and adds them to the splats'
 // 2-D gradient sums.  Lane l works on collected splat c = l / 16, block
 // column s = l % 8 and half h = (l / 8) % 2 of the rows (pixels s + 8 i for
 // i = 4h .. 4h + 3: dx is constant per lane); the 16 lanes of a splat then
-// combine their partial sums with four DPP steps and add the nine totals
-// with one atomic instruction (lane s of the first half the total s, the
-// first lane of the second half the ninth): the instruction's lanes address
-// one splat's entry each, as the atomics' cost is per addressed cache line.
+// combine their nine partial sums by a reduce-scatter of four DPP steps
+// (partners l ^ 8, l ^ 7, l ^ 2, l ^ 1): at each step a lane keeps about
+// half of its values, adding the partner's copy of them, so the step
+// costs about as many adds as the lane keeps values (13, 7, 4 and 3
+// instructions with the selects, against 9 per step for a full reduction of
+// every value).  Nine lanes end with one total each and add it with one
+// atomic instruction, whose lanes address one splat's entry each (the
+// atomics' cost is per addressed cache line).
 // slots: 6 bits per collected splat, its slot in the staged batch.
 __device__ __forceinline__ void backward_flush(BwdWaveLds& L, uint32_t n, uint64_t slots, uint32_t x0,
                                                uint32_t y0, uint32_t splats, float* __restrict__ grad2d) {
@@ -108,33 +117,39 @@ __device__ __forceinline__ void backward_flush(BwdWaveLds& L, uint32_t n, uint64
             sb = __builtin_fmaf(pr.y, gv.z, sb);
         }
     }
-    // moments with the lane's constant dx: sum g dx = dx sum g, and so on
-    float v[9] = {dx * sg, sy, dx * dx * sg, dx * sy, syy, sg, sr, sgg, sb};
-#pragma unroll
-    for (int k = 0; k < 9; ++k) {
-        float t = v[k];
-        t += dpp_f32<0xB1>(t);   // quad_perm [1,0,3,2]
-        t += dpp_f32<0x4E>(t);   // quad_perm [2,3,0,1]
-        t += dpp_f32<0x141>(t);  // row_half_mirror: the 8 columns' total
-        t += dpp_f32<0x128>(t);  // row_ror:8: plus the other half's, in all 16 lanes
-        v[k] = t;
-    }
+    // moments with the lane's constant dx: sum g dx = dx sum g, and so on;
     // opacity: sum of e gal over the pixels off the 0.99 clamp = sum gpow / op
     // (v_rcp_f32, 1 ulp, instead of an IEEE division sequence)
-    v[5] = v[5] * __builtin_amdgcn_rcpf(op);
+    const float v[9] = {dx * sg, sy, dx * dx * sg, dx * sy, syy, sg * __builtin_amdgcn_rcpf(op), sr, sgg, sb};
     const bool b0 = col & 1, b1 = col & 2, b2 = col & 4;
-    const float s01 = b0 ? v[1] : v[0], s23 = b0 ? v[3] : v[2], s45 = b0 ? v[5] : v[4], s67 = b0 ? v[7] : v[6];
-    const float s03 = b1 ? s23 : s01, s47 = b1 ? s67 : s45;
-    const float first8 = b2 ? s47 : s03;
-    const float mine = half ? v[8] : first8;
+    // partner l ^ 8 (row_ror:8 within the 16-lane row): the first half keeps
+    // totals 0-4, the second 5-8
+    float w[5];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const float keep = half ? v[k + 5] : v[k], send = half ? v[k] : v[k + 5];
+        w[k] = keep + dpp_f32<0x128>(send);
+    }
+    w[4] = v[4] + dpp_f32<0x128>(v[4]);
+    // partner l ^ 7 (row_half_mirror): columns 0-3 keep w0-w2, 4-7 w3, w4
+    float x[3];
+    x[0] = (b2 ? w[3] : w[0]) + dpp_f32<0x141>(b2 ? w[0] : w[3]);
+    x[1] = (b2 ? w[4] : w[1]) + dpp_f32<0x141>(b2 ? w[1] : w[4]);
+    x[2] = w[2] + dpp_f32<0x141>(w[2]);
+    // partner l ^ 2 (quad_perm [2,3,0,1])
+    const float y0v = (b1 ? x[2] : x[0]) + dpp_f32<0x4E>(b1 ? x[0] : x[2]);
+    const float y1v = x[1] + dpp_f32<0x4E>(x[1]);
+    // partner l ^ 1 (quad_perm [1,0,3,2])
+    const float mine = (b0 ? y1v : y0v) + dpp_f32<0xB1>(b0 ? y0v : y1v);
     // the totals are formed by every lane before the branch below: sunk into
     // it, the last DPP step could not fold into its add
     asm volatile("" ::"v"(mine));
     // the collected slots are reused by the next chunk
     __builtin_amdgcn_wave_barrier();
-    if (c < n && gid < splats && (half == 0 || col == 0)) {
+    const uint32_t slot = (uint32_t)(kFlushTotalOfLane >> (4 * (lane & 15))) & 0xFu;
+    if (c < n && gid < splats && slot != 0xFu) {
         float* dst = grad2d + (size_t)gid * 12;
-        unsafeAtomicAdd(dst + (half ? 8u : col), mine);
+        unsafeAtomicAdd(dst + slot, mine);
     }
 }
 
