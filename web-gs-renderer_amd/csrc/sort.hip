@@ -281,17 +281,18 @@ __device__ __forceinline__ void onesweep_tiles(OnesweepLds<K, THREADS, RB>& L, c
             // peers: the lanes whose digit equals this lane's, one ballot per
             // digit bit, unrolled over the digit width (a narrower last pass
             // has zero high bits everywhere, which keep every lane): keep the
-            // lanes that agree, bb where this lane's bit is 1 and ~bb where it
-            // is 0, i.e. bb ^ (bit - 1)
+            // lanes that agree, m &= ~(bb ^ bitv) with bitv the lane's bit
+            // sign-extended to a word (one bit-field extract; the ballot is a
+            // compare of it; the update one three-input bit operation per
+            // half of the mask)
             const uint64_t okm = ballot(ok);
             uint32_t m_lo = (uint32_t)okm, m_hi = (uint32_t)(okm >> 32);
 #pragma unroll
             for (int b = 0; b < RB; ++b) {
-                const uint32_t bit = (d >> b) & 1u;
-                const uint64_t bb = ballot(bit != 0u);
-                const uint32_t flip = bit - 1u;
-                m_lo &= (uint32_t)bb ^ flip;
-                m_hi &= (uint32_t)(bb >> 32) ^ flip;
+                const uint32_t bitv = (uint32_t)((int32_t)(d << (31 - b)) >> 31);
+                const uint64_t bb = ballot(bitv != 0u);
+                m_lo &= ~((uint32_t)bb ^ bitv);
+                m_hi &= ~((uint32_t)(bb >> 32) ^ bitv);
             }
             const uint64_t m = (uint64_t)m_hi << 32 | m_lo;
             const uint32_t below = count_below(m);
