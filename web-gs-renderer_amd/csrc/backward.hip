@@ -40,30 +40,26 @@ __device__ __forceinline__ float dpp_f32(float v) {
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
 }
 
-// One gathered splat as staged in LDS (colour w carries the splat index).
-struct SplatB {
-    float4 co;  // conic A, B, C and opacity
-    float4 cl;  // colour, splat index bits
-    float2 p;   // pixel position
-    float2 pad;
-};
-
 // Splats whose per-pixel terms a wave collects before it sums them per
 // splat (see backward_block).
 constexpr int kBwdChunk = 4;
 
-// LDS of one wave of backward_block (the workgroup holds 4; 6 KiB each, so
-// that six workgroups, 24 waves, fit a CU: with 8 KiB per wave and 8
-// collected splats five fitted, and the backward raster took 0.369 ms
-// instead of 0.292 at 1M splats and 1080p).  pair[c] holds collected splat
-// c's terms with pixel p at p ^ 8c, so that the flush's reads of different
-// splats fall in different banks.
+// LDS of one wave of backward_block (the workgroup holds 4; 5.5 KiB each, so
+// that seven workgroups, 28 waves, fit a CU: with the staged records padded
+// to 48 bytes (6 KiB per wave) six fitted, and with 8 KiB per wave and 8
+// collected splats five).  The staged batch is three arrays (conic and
+// opacity; colour with the splat index in w; pixel position), each read
+// back with one same-address load.  pair[c] holds collected splat c's terms
+// with pixel p at p ^ 8c, so that the flush's reads of different splats fall
+// in different banks.
 struct BwdWaveLds {
-    SplatB spl[64];                 // the staged batch
+    float4 co[64];                  // the staged batch: conic A, B, C and opacity
+    float4 cl[64];                  //   colour, splat index bits
+    float2 p[64];                   //   pixel position
     float2 pair[kBwdChunk][64];     // per collected splat and pixel: (gpow, w)
     float4 g[64];                   // per pixel: dL/d(r, g, b)
 };
-static_assert(sizeof(BwdWaveLds) == 6144, "backward LDS per wave");
+static_assert(sizeof(BwdWaveLds) == 5632, "backward LDS per wave");
 
 // Which of the nine per-splat totals lane l16 = lane % 16 of a splat's
 // 16-lane group holds after backward_flush's reduce-scatter (4 bits per
@@ -96,10 +92,11 @@ __device__ __forceinline__ void backward_flush(BwdWaveLds& L, uint32_t n, uint64
     float op = 1.f;
     uint32_t gid = 0xFFFFFFFFu;
     if (c < n) {
-        const SplatB& sp = L.spl[(uint32_t)(slots >> (6 * c)) & 63u];
-        const float ix = sp.p.x, iy = sp.p.y;
-        op = sp.co.w;
-        gid = __float_as_uint(sp.cl.w);
+        const uint32_t sl = (uint32_t)(slots >> (6 * c)) & 63u;
+        const float2 sp = L.p[sl];
+        const float ix = sp.x, iy = sp.y;
+        op = L.co[sl].w;
+        gid = __float_as_uint(L.cl[sl].w);
         dx = ix - (float)(x0 + col);
         const float fy0 = (float)y0;
 #pragma unroll
@@ -216,9 +213,9 @@ __device__ __forceinline__ void backward_block(uint32_t W, uint32_t H, uint32_t 
         const bool live = valid && reaches_rect(p.x, p.y, co.x, co.y, co.z, sr.pq.z, rxa, rxb, rya, ryb);
         // Stage the batch in this wave's LDS slots (as in the forward).
         __builtin_amdgcn_wave_barrier();
-        L.spl[lane].co = co;
-        L.spl[lane].cl = make_float4(cl.x, cl.y, cl.z, __uint_as_float(gid));
-        L.spl[lane].p = p;
+        L.co[lane] = co;
+        L.cl[lane] = make_float4(cl.x, cl.y, cl.z, __uint_as_float(gid));
+        L.p[lane] = p;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -230,10 +227,11 @@ __device__ __forceinline__ void backward_block(uint32_t W, uint32_t H, uint32_t 
             const int j = 63 - (int)__builtin_clzll(lm);
             lm &= ~(1ull << j);
             const uint32_t pos = base + (uint32_t)j;
-            const SplatB s = L.spl[j];
-            const float A = s.co.x, B = s.co.y, C = s.co.z, op = s.co.w;  // A..C: power only
-            const float cr = s.cl.x, cg = s.cl.y, cb = s.cl.z;
-            const float dx = s.p.x - fx, dy = s.p.y - fy;
+            const float4 sco = L.co[j], scl = L.cl[j];
+            const float2 sp = L.p[j];
+            const float A = sco.x, B = sco.y, C = sco.z, op = sco.w;  // A..C: power only
+            const float cr = scl.x, cg = scl.y, cb = scl.z;
+            const float dx = sp.x - fx, dy = sp.y - fy;
             const float q = __builtin_fmaf(A * dx, dx, (C * dy) * dy);
             const float power = __builtin_fmaf(-0.5f, q, -((B * dx) * dy));
             const float e = exp_blend(power);
