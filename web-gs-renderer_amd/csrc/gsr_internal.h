@@ -86,7 +86,7 @@ __device__ __forceinline__ float gsr_exp(float x) {
 //   pq = (pixel x, pixel y, q_max, 0): q_max the widened reach threshold of
 //        cull.h, computed once per splat
 //   cl = (r, g, b, 0)
-//   pad = 0
+//   pad = 0 (left unwritten in view batches; no kernel reads it)
 // The record of a splat that touches no tile is never read.
 struct alignas(64) SplatRec {
     float4 co, pq, cl, pad;

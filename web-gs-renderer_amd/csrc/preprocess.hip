@@ -193,6 +193,17 @@ __device__ __forceinline__ SplatRec make_rec(const float4& conic, const float2& 
     return r;
 }
 
+// The batch path's record store: the 16 bytes of padding are left unwritten
+// (no kernel reads them).  6M splats x 8 views: 768 MB fewer bytes written,
+// preprocess 1.83 -> 1.77 ms; the plain frame keeps the full store, which
+// was 2 us faster there.
+__device__ __forceinline__ void store_rec_unpadded(SplatRec* __restrict__ dst, const float4& conic, const float2& xy,
+                                                   const float4& rgb) {
+    dst->co = conic;
+    dst->pq = make_float4(xy.x, xy.y, reach_qmax(conic.w), 0.0f);
+    dst->cl = rgb;
+}
+
 // PACKED: write records (rec_out) instead of the three arrays.
 template <bool PACKED>
 __global__ __launch_bounds__(256) void k_preprocess(const float* __restrict__ P, uint32_t n, CamK cam,
@@ -289,7 +300,7 @@ __global__ __launch_bounds__(256) void k_preprocess_batch(const float* __restric
         const size_t o = (size_t)v * n + g;
         tiles_out[o] = ntiles;
         meta_out[o] = meta;
-        rec_out[o] = make_rec(conic, xy, rgb);
+        store_rec_unpadded(rec_out + o, conic, xy, rgb);
     }
 }
 
