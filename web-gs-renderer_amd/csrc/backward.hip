@@ -162,7 +162,7 @@ __device__ __forceinline__ void backward_flush(BwdWaveLds& L, uint32_t n, uint64
 // gpow (g dx, g dy, g dx^2, g dx dy, g dy^2), the opacity term and the colour
 // terms, added to grad2d per entry of vals.  Summing per splat over block
 // columns replaces a 64-lane reduction per splat and block by a few adds per
-// pixel and four DPP steps per sum.
+// pixel and a four-step DPP reduce-scatter of the nine sums over 16 lanes.
 __device__ __forceinline__ void backward_block(uint32_t W, uint32_t H, uint32_t x0, uint32_t y0, uint32_t start,
                                                uint32_t range_end, uint32_t splats, const uint32_t* __restrict__ vals,
                                                const SplatRec* __restrict__ rec, const float* __restrict__ tfinal,
