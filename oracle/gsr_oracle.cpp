@@ -21,7 +21,8 @@
 // where written, IEEE sqrt and division, exp from exp_f32() below
 // (Cody-Waite reduction, degree-7 Taylor polynomial) in preprocess and from
 // exp_blend_f32() (exp2 reduction, degree-5 polynomial, for powers in
-// [-5.55, 0]) in the blend.  The HIP kernels follow the same order, so their
+// [-5.55, 0]) in the blend, whose power and transmittance follow
+// power_f32() and transmit_f32().  The HIP kernels follow the same order, so their
 // FP32 outputs are comparable bit for bit.  FP64 versions (libm exp, same
 // formulas) measure the FP32 error.
 //
@@ -123,18 +124,37 @@ inline float exp_blend_f32(float p) {
 // which is what pins this cut (tests/test_oracle.py).
 inline bool blend_range_f32(float power) { return !(power > 0.0f) && !(power < -5.55f); }
 
+// The FP32 power of a splat at pixel offset (dx, dy) = (splat - pixel):
+// calculatePower (raseter.wgsl:113-117), -0.5 (A dx^2 + C dy^2) - B dx dy,
+// evaluated as dx (a dx + b dy) + c dy^2 with a = -A/2, b = -B, c = -C/2
+// (scalings by -1/2 and -1, exact), two FMAs and three products, the order
+// the kernels use.
+inline float power_f32(float A, float B, float C, float dx, float dy) {
+    return fmaf(dx, fmaf(-0.5f * A, dx, (-B) * dy), ((-0.5f * C) * dy) * dy);
+}
+// The transmittance after blending alpha: T (1 - alpha) (raseter.wgsl:93) as
+// one FMA, T - alpha T, the kernels' order.
+inline float transmit_f32(float T, float alpha) { return fmaf(-alpha, T, T); }
+
 template <typename T> struct M;
 template <> struct M<float> {
     static float ex(float x) { return exp_f32(x); }
     static float ex_blend(float x) { return exp_blend_f32(x); }
     static bool in_blend_range(float p) { return blend_range_f32(p); }
     static float fma(float a, float b, float c) { return fmaf(a, b, c); }
+    static float power(float A, float B, float C, float dx, float dy) { return power_f32(A, B, C, dx, dy); }
+    static float transmit(float T, float alpha) { return transmit_f32(T, alpha); }
 };
 template <> struct M<double> {
     static double ex(double x) { return std::exp(x); }
     static double ex_blend(double x) { return std::exp(x); }
     static bool in_blend_range(double p) { return !(p > 0.0); }
     static double fma(double a, double b, double c) { return std::fma(a, b, c); }
+    // the reference's formulas as written
+    static double power(double A, double B, double C, double dx, double dy) {
+        return -0.5 * (A * dx * dx + C * dy * dy) - B * dx * dy;
+    }
+    static double transmit(double T, double alpha) { return T * (1.0 - alpha); }
 };
 
 // SH constants (preprocess.wgsl:41-54).
@@ -469,12 +489,11 @@ inline void blend_pixel(T fxp, T fyp, uint32_t len, Splat splat, Id id, T& c0, T
         const SplatV<T> s = splat(i);
         const T dx = s.x - fxp, dy = s.y - fyp;
         // power = -0.5 (A dx^2 + C dy^2) - B dx dy  (calculatePower :113-117)
-        const T q = M<T>::fma(s.A * dx, dx, (s.C * dy) * dy);
-        const T power = M<T>::fma(T(-0.5), q, -((s.B * dx) * dy));
+        const T power = M<T>::power(s.A, s.B, s.C, dx, dy);
         if (!M<T>::in_blend_range(power)) continue;
         const T alpha = std::min(T(0.99f), s.op * M<T>::ex_blend(power));
         if (alpha < T(1.0f / 255.0f)) continue;
-        const T testT = Tr * (T(1) - alpha);
+        const T testT = M<T>::transmit(Tr, alpha);
         if (testT < T(0.0001f)) {
             h = fnv(h, 0xFFFFFFFFull + i);
             break;
@@ -837,13 +856,12 @@ void backward_f64(const float* packed, uint32_t n, const Cam& cam, uint32_t tile
                             const uint32_t g = vals[i];
                             const float A = conicf[g * 4], B = conicf[g * 4 + 1], C = conicf[g * 4 + 2];
                             const float dx = xyf[g * 2] - (float)x, dy = xyf[g * 2 + 1] - (float)y;
-                            const float q = fmaf(A * dx, dx, (C * dy) * dy);
-                            const float power = fmaf(-0.5f, q, -((B * dx) * dy));
+                            const float power = power_f32(A, B, C, dx, dy);
                             if (!blend_range_f32(power)) continue;
                             const float oe = conicf[g * 4 + 3] * exp_blend_f32(power);
                             const float alpha = std::min(0.99f, oe);
                             if (alpha < 1.0f / 255.0f) continue;
-                            const float tn = Tr * (1.0f - alpha);
+                            const float tn = transmit_f32(Tr, alpha);
                             if (tn < 0.0001f) break;
                             Tr = tn;
                             blend.push_back(i);
@@ -1121,8 +1139,7 @@ void raster_backward_f32(uint32_t W, uint32_t H, uint32_t tile, const uint32_t* 
                     const uint32_t g = vals[start + pos - 1];
                     const float A = conic[g * 4], B = conic[g * 4 + 1], C = conic[g * 4 + 2], op = conic[g * 4 + 3];
                     const float dx = xy[g * 2] - (float)x, dy = xy[g * 2 + 1] - (float)y;
-                    const float q = fmaf(A * dx, dx, (C * dy) * dy);
-                    const float power = fmaf(-0.5f, q, -((B * dx) * dy));
+                    const float power = power_f32(A, B, C, dx, dy);
                     if (!blend_range_f32(power)) continue;
                     const float e = exp_blend_f32(power);
                     const float oe = op * e;

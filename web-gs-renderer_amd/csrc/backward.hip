@@ -53,7 +53,7 @@ constexpr int kBwdChunk = 4;
 // with pixel p at p ^ 8c, so that the flush's reads of different splats fall
 // in different banks.
 struct BwdWaveLds {
-    float4 co[64];                  // the staged batch: conic A, B, C and opacity
+    float4 co[64];                  // the staged batch: -A/2, -B, -C/2 and opacity
     float4 cl[64];                  //   colour, splat index bits
     float2 p[64];                   //   pixel position
     float2 pair[kBwdChunk][64];     // per collected splat and pixel: (gpow, w)
@@ -213,7 +213,7 @@ __device__ __forceinline__ void backward_block(uint32_t W, uint32_t H, uint32_t 
         const bool live = valid && reaches_rect(p.x, p.y, co.x, co.y, co.z, sr.pq.z, rxa, rxb, rya, ryb);
         // Stage the batch in this wave's LDS slots (as in the forward).
         __builtin_amdgcn_wave_barrier();
-        L.co[lane] = co;
+        L.co[lane] = make_float4(-0.5f * co.x, -co.y, -0.5f * co.z, co.w);  // as in the forward
         L.cl[lane] = make_float4(cl.x, cl.y, cl.z, __uint_as_float(gid));
         L.p[lane] = p;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -229,11 +229,10 @@ __device__ __forceinline__ void backward_block(uint32_t W, uint32_t H, uint32_t 
             const uint32_t pos = base + (uint32_t)j;
             const float4 sco = L.co[j], scl = L.cl[j];
             const float2 sp = L.p[j];
-            const float A = sco.x, B = sco.y, C = sco.z, op = sco.w;  // A..C: power only
+            const float A = sco.x, B = sco.y, C = sco.z, op = sco.w;  // -A/2, -B, -C/2: power only
             const float cr = scl.x, cg = scl.y, cb = scl.z;
             const float dx = sp.x - fx, dy = sp.y - fy;
-            const float q = __builtin_fmaf(A * dx, dx, (C * dy) * dy);
-            const float power = __builtin_fmaf(-0.5f, q, -((B * dx) * dy));
+            const float power = __builtin_fmaf(dx, __builtin_fmaf(A, dx, B * dy), (C * dy) * dy);
             const float e = exp_blend(power);
             const float oe = op * e;
             const float alpha = fminf(kAlphaClamp, oe);

@@ -79,7 +79,7 @@ __global__ __launch_bounds__(256) void k_zero_regions(ZeroRegions z) {
 // One gathered splat as staged in LDS: 48 bytes, read back with three
 // same-address loads (the LDS broadcasts them to the wave).
 struct SplatL {
-    float4 co;  // conic A, B, C and opacity
+    float4 co;  // -A/2, -B, -C/2 of the conic, and opacity
     float4 cl;  // colour (w unused)
     float2 p;   // pixel position
     float2 pad;
@@ -125,8 +125,10 @@ __device__ __forceinline__ void raster_block(uint32_t W, uint32_t H, uint32_t x0
         // Stage the batch in this wave's LDS slots.  Only this wave touches
         // them: a wave-level barrier orders the previous batch's reads before
         // these writes and these writes before the reads below.
+        // The conic is staged as (-A/2, -B, -C/2) (exact scalings), so that
+        // the power is dx (a dx + b dy) + c dy^2: two FMAs and three products
         __builtin_amdgcn_wave_barrier();
-        s_spl[wv][lane].co = co;
+        s_spl[wv][lane].co = make_float4(-0.5f * co.x, -co.y, -0.5f * co.z, co.w);
         s_spl[wv][lane].cl = cl;
         s_spl[wv][lane].p = p;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -145,10 +147,9 @@ __device__ __forceinline__ void raster_block(uint32_t W, uint32_t H, uint32_t x0
             const int j = (int)__builtin_ctzll(lm);
             const SplatL s = s_spl[wv][j];
             const float dx = s.p.x - fx, dy = s.p.y - fy;
-            const float q = __builtin_fmaf(s.co.x * dx, dx, (s.co.z * dy) * dy);
-            const float power = __builtin_fmaf(-0.5f, q, -((s.co.y * dx) * dy));
+            const float power = __builtin_fmaf(dx, __builtin_fmaf(s.co.x, dx, s.co.y * dy), (s.co.z * dy) * dy);
             const float alpha = fminf(kAlphaClamp, s.co.w * exp_blend(power));
-            const float tn = T * (1.0f - alpha);
+            const float tn = __builtin_fmaf(-alpha, T, T);  // T (1 - alpha) as T - alpha T
             const bool ok = !done & !(power > 0.0f) & !(power < -5.55f) & !(alpha < kAlphaMin);
             done |= ok & (tn < kTransmitMin);
             // ok lanes were not done before: blended where still not done
