@@ -30,7 +30,9 @@
 // within 2.7 ulp on [-5.55, 0] and its ldexp equal, on every float there, to
 // the kernels' exponent insertion by shift-and-add; the FP32 blend decision (blend_range_f32 and
 // the alpha tests) against FP64 libm over every power from 0 down to -inf,
-// and a raster of far thin splats against the FP64 raster; the integer
+// and a raster of far thin splats against the FP64 raster; power_f32 exact on
+// small integers and within 6 FP32 roundings of calculatePower in FP64,
+// transmit_f32 bit-equal to the rounded exact T (1 - alpha); the integer
 // chain against closed forms / brute force; preprocess special cases
 // (isotropic splat, on-axis Jacobian, SH degree 0, clamp, sigmoid(0));
 // FP32 vs FP64 agreement; raster closed forms on 1-3 splats.
@@ -1227,6 +1229,17 @@ void orc_blend_alpha_f32(const float* power, const float* op, float* alpha, int6
         }
         alpha[i] = a;
     }
+}
+
+// The FP32 blend's power and transmittance update, elementwise (pinned
+// against the FP64 formulas in tests/test_oracle.py).
+void orc_power_f32_array(const float* A, const float* B, const float* C, const float* dx, const float* dy,
+                         float* power, int64_t n) {
+    for (int64_t i = 0; i < n; ++i) power[i] = power_f32(A[i], B[i], C[i], dx[i], dy[i]);
+}
+
+void orc_transmit_f32_array(const float* T, const float* alpha, float* out, int64_t n) {
+    for (int64_t i = 0; i < n; ++i) out[i] = transmit_f32(T[i], alpha[i]);
 }
 
 void orc_exp_f32_array(const float* x, float* y, int64_t n) {

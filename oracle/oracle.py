@@ -36,6 +36,8 @@ def lib():
         L.orc_exp_f32_array.argtypes = [_f32p, _f32p, ctypes.c_int64]
         L.orc_exp_blend_f32_array.argtypes = [_f32p, _f32p, ctypes.c_int64]
         L.orc_blend_alpha_f32.argtypes = [_f32p, _f32p, _f32p, ctypes.c_int64]
+        L.orc_power_f32_array.argtypes = [_f32p, _f32p, _f32p, _f32p, _f32p, _f32p, ctypes.c_int64]
+        L.orc_transmit_f32_array.argtypes = [_f32p, _f32p, _f32p, ctypes.c_int64]
         L.orc_exp_blend_shift_mismatches.restype = ctypes.c_int64
         L.orc_exp_blend_shift_mismatches.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
         L.orc_num_threads.restype = ctypes.c_int
@@ -98,6 +100,25 @@ def blend_alpha_f32(power, op):
     a = np.empty_like(p)
     lib().orc_blend_alpha_f32(p.ravel(), o.ravel(), a.ravel(), p.size)
     return a
+
+
+def power_f32(A, B, C, dx, dy):
+    """The FP32 blend's power of a splat with conic (A, B, C) at pixel offset
+    (dx, dy) = (splat - pixel), elementwise, in the kernels' order."""
+    arrs = [np.ascontiguousarray(a, np.float32).ravel() for a in np.broadcast_arrays(
+        *[np.asarray(v, np.float32) for v in (A, B, C, dx, dy)])]
+    out = np.empty_like(arrs[0])
+    lib().orc_power_f32_array(*arrs, out, out.size)
+    return out
+
+
+def transmit_f32(T, alpha):
+    """The FP32 blend's transmittance after blending alpha, elementwise."""
+    t, a = [np.ascontiguousarray(v, np.float32).ravel() for v in np.broadcast_arrays(
+        np.asarray(T, np.float32), np.asarray(alpha, np.float32))]
+    out = np.empty_like(t)
+    lib().orc_transmit_f32_array(t, a, out, out.size)
+    return out
 
 
 def exp_blend_shift_mismatches(mag_lo, mag_hi):

@@ -48,6 +48,48 @@ def test_exp_blend_matches_libm_on_the_blend_range(orc):
     assert (np.diff(y) >= 0).all()
 
 
+def test_blend_power_matches_calculate_power(orc):
+    """power_f32 (dx (a dx + b dy) + c dy^2, a = -A/2, b = -B, c = -C/2) is
+    calculatePower's -0.5 (A dx^2 + C dy^2) - B dx dy (raseter.wgsl:113-117):
+    exact on small integers (a wrong sign of B, a swapped dx/dy or a lost
+    half changes these), and within a few FP32 roundings of the FP64 formula
+    on random positive-definite conics, near and far from the centre."""
+    A = np.array([1, 2, 1, 1, 1, 3], np.float32)
+    B = np.array([0, 0.5, 1, 0, -1, 0.25], np.float32)
+    C = np.array([1, 1, 1, 4, 2, 0.5], np.float32)
+    dx = np.array([3, 1, 1, 1, 2, -4], np.float32)
+    dy = np.array([4, 2, 1, 0, -1, 2], np.float32)
+    want = -0.5 * (A.astype(np.float64) * dx * dx + C.astype(np.float64) * dy * dy) - B.astype(np.float64) * dx * dy
+    assert np.array_equal(orc.power_f32(A, B, C, dx, dy), want.astype(np.float32))
+    assert orc.power_f32(1, 0, 4, 1, 0)[0] == -0.5 and orc.power_f32(1, 0, 4, 0, 1)[0] == -2.0
+    rng = np.random.default_rng(11)
+    n = 1_000_000
+    A = rng.uniform(1e-3, 3.4, n).astype(np.float32)
+    C = rng.uniform(1e-3, 3.4, n).astype(np.float32)
+    B = (rng.uniform(-0.99, 0.99, n) * np.sqrt(A.astype(np.float64) * C)).astype(np.float32)
+    scale = np.where(rng.random(n) < 0.5, 4.0, 300.0)
+    dx = (rng.uniform(-1, 1, n) * scale).astype(np.float32)
+    dy = (rng.uniform(-1, 1, n) * scale).astype(np.float32)
+    p32 = orc.power_f32(A, B, C, dx, dy).astype(np.float64)
+    A64, B64, C64, dx64, dy64 = (v.astype(np.float64) for v in (A, B, C, dx, dy))
+    p64 = -0.5 * (A64 * dx64 * dx64 + C64 * dy64 * dy64) - B64 * dx64 * dy64
+    mag = 0.5 * A64 * dx64 ** 2 + np.abs(B64 * dx64 * dy64) + 0.5 * C64 * dy64 ** 2
+    assert (np.abs(p32 - p64) <= 6 * 2.0 ** -24 * mag + 1e-30).all()
+    assert (p64 <= 0).all()
+
+
+def test_blend_transmit_is_the_rounded_product(orc):
+    """transmit_f32 (T - alpha T in one FMA) is T (1 - alpha) (raseter.wgsl:93)
+    rounded once: bit-equal to the FP32 rounding of the exact product, which
+    FP64 holds exactly for FP32 operands."""
+    rng = np.random.default_rng(12)
+    T = rng.uniform(1e-4, 1.0, 1_000_000).astype(np.float32)
+    a = rng.uniform(1 / 255, 0.99, 1_000_000).astype(np.float32)
+    exact = (T.astype(np.float64) * (1.0 - a.astype(np.float64))).astype(np.float32)
+    assert np.array_equal(orc.transmit_f32(T, a), exact)
+    assert orc.transmit_f32(0.5, 0.25)[0] == np.float32(0.375)
+
+
 def _blend_alpha_f64(power, op):
     """raseter.wgsl:84-92 in FP64 with libm exp: skip (0) when power > 0 or alpha < 1/255."""
     p = np.asarray(power, np.float64)
