@@ -425,9 +425,9 @@ __device__ __forceinline__ void splat_shape(const Par& par, SplatShape& s) {
 //   gch   dL/d(SH colour) per channel (zero where the forward clamped it);
 //   basis the 15 higher SH basis values of the view direction;
 // and the view direction's part of dL/dposition is added to gp.
-template <typename Par, typename Shape>
+template <bool POS_MOMENTS = false, typename Par, typename Shape>
 __device__ __forceinline__ bool view_adjoint(const Par& par, const Shape& s,
-                                             const CamK& cam, const float (&Gv)[9], float (&gp)[3],
+                                             const CamK& cam, float (&Gv)[9], float (&gp)[3],
                                              float (&gsym)[3][3], float (&gch)[3], float (&basis)[15]) {
     const float px = s.px, py = s.py, pz = s.pz;
     const float* V = cam.view;
@@ -436,18 +436,6 @@ __device__ __forceinline__ bool view_adjoint(const Par& par, const Shape& s,
     const float ty = ((V[1] * px + V[5] * py) + V[9] * pz) + V[13];
     const float tz = ((V[2] * px + V[6] * py) + V[10] * pz) + V[14];
     if (!(tz < -kNearView)) return false;
-    {   // pixel position <- ndc <- clip <- p
-        const float cx = ((PV[0] * px + PV[4] * py) + PV[8] * pz) + PV[12];
-        const float cy = ((PV[1] * px + PV[5] * py) + PV[9] * pz) + PV[13];
-        const float cw = ((PV[3] * px + PV[7] * py) + PV[11] * pz) + PV[15];
-        const float w = cw + kSmall;
-        const float gnx = Gv[0] * cam.texture_size[0] * 0.5f, gny = Gv[1] * cam.texture_size[1] * 0.5f;
-        const float iw = 1.0f / w;
-        const float gcx = gnx * iw, gcy = gny * iw, gcw = -(gnx * cx + gny * cy) * iw * iw;
-        gp[0] = PV[0] * gcx + PV[1] * gcy + PV[3] * gcw;
-        gp[1] = PV[4] * gcx + PV[5] * gcy + PV[7] * gcw;
-        gp[2] = PV[8] * gcx + PV[9] * gcy + PV[11] * gcw;
-    }
     // covariance chain
     const float limx = kFrustumGuard * cam.tan_fov[0], limy = kFrustumGuard * cam.tan_fov[1];
     const float vx = tx / tz, vy = ty / tz;
@@ -474,6 +462,27 @@ __device__ __forceinline__ bool view_adjoint(const Par& par, const Shape& s,
     const float b = Tm[0][0] * St1[0] + Tm[0][1] * St1[1] + Tm[0][2] * St1[2];
     const float c = Tm[1][0] * St1[0] + Tm[1][1] * St1[1] + Tm[1][2] * St1[2] + 0.3f;
     const float det = a * c - b * b;
+    if (POS_MOMENTS) {
+        // Gv[0], Gv[1] arrive as the moments sum g dx, sum g dy; the pixel
+        // position's gradient needs the conic, recomputed here from this
+        // chain (c, -b, a) / det instead of read from the forward's record
+        const float A = c / det, B = -b / det, C = a / det;
+        const float mx = Gv[0], my = Gv[1];
+        Gv[0] = -__builtin_fmaf(A, mx, B * my);
+        Gv[1] = -__builtin_fmaf(C, my, B * mx);
+    }
+    {   // pixel position <- ndc <- clip <- p
+        const float cx = ((PV[0] * px + PV[4] * py) + PV[8] * pz) + PV[12];
+        const float cy = ((PV[1] * px + PV[5] * py) + PV[9] * pz) + PV[13];
+        const float cw = ((PV[3] * px + PV[7] * py) + PV[11] * pz) + PV[15];
+        const float w = cw + kSmall;
+        const float gnx = Gv[0] * cam.texture_size[0] * 0.5f, gny = Gv[1] * cam.texture_size[1] * 0.5f;
+        const float iw = 1.0f / w;
+        const float gcx = gnx * iw, gcy = gny * iw, gcw = -(gnx * cx + gny * cy) * iw * iw;
+        gp[0] = PV[0] * gcx + PV[1] * gcy + PV[3] * gcw;
+        gp[1] = PV[4] * gcx + PV[5] * gcy + PV[7] * gcw;
+        gp[2] = PV[8] * gcx + PV[9] * gcy + PV[11] * gcw;
+    }
     const float id2 = 1.0f / (det * det);
     const float gA = Gv[2], gB = Gv[3], gC = Gv[4];
     const float ga = (gA * (-c * c) + gB * (b * c) + gC * (-b * b)) * id2;
@@ -718,14 +727,18 @@ __global__ __launch_bounds__(256) void k_preprocess_backward_batch(const float* 
         const float4 G0 = grad2d[e * 3], G1 = grad2d[e * 3 + 1], G2 = grad2d[e * 3 + 2];
         const bool any = (G0.x != 0.0f) | (G0.y != 0.0f) | (G0.z != 0.0f) | (G0.w != 0.0f) | (G1.x != 0.0f) |
                          (G1.y != 0.0f) | (G1.z != 0.0f) | (G1.w != 0.0f) | (G2.x != 0.0f);
-        float Gv[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-        if (any) moments_to_2d(G0, G1, G2, rec[e].co, Gv);
-        if (grad2d_out) {
-            grad2d_out[e * 3] = make_float4(Gv[0], Gv[1], Gv[2], Gv[3]);
-            grad2d_out[e * 3 + 1] = make_float4(Gv[4], Gv[5], Gv[6], Gv[7]);
-            grad2d_out[e * 3 + 2] = make_float4(Gv[8], 0.0f, 0.0f, 0.0f);
+        // the 2-D gradient with the position's two moments left as they are:
+        // view_adjoint turns them into d/dx, d/dy with the conic it
+        // recomputes (no read of the forward's record)
+        float Gv[9] = {G0.x, G0.y, -0.5f * G0.z, -G0.w, -0.5f * G1.x, G1.y, G1.z, G1.w, G2.x};
+        if (!any) {
+            if (grad2d_out) {
+                grad2d_out[e * 3] = make_float4(0.f, 0.f, 0.f, 0.f);
+                grad2d_out[e * 3 + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
+                grad2d_out[e * 3 + 2] = make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+            continue;
         }
-        if (!any) continue;
         const float* Pv = opaque(P);  // the shape read where it is needed, not held through the loop
         if (!shaped) {
             SplatShape full;
@@ -740,7 +753,13 @@ __global__ __launch_bounds__(256) void k_preprocess_backward_batch(const float* 
             shaped = true;
         }
         float gp[3], gsym[3][3], gch[3], basis[15];
-        if (!view_adjoint(ParShLds{Pv, N, g, s_sh, col}, cv, cams[v], Gv, gp, gsym, gch, basis)) continue;
+        const bool ok = view_adjoint<true>(ParShLds{Pv, N, g, s_sh, col}, cv, cams[v], Gv, gp, gsym, gch, basis);
+        if (grad2d_out) {
+            grad2d_out[e * 3] = make_float4(ok ? Gv[0] : 0.f, ok ? Gv[1] : 0.f, Gv[2], Gv[3]);
+            grad2d_out[e * 3 + 1] = make_float4(Gv[4], Gv[5], Gv[6], Gv[7]);
+            grad2d_out[e * 3 + 2] = make_float4(Gv[8], 0.0f, 0.0f, 0.0f);
+        }
+        if (!ok) continue;
         touched = true;
 #pragma unroll
         for (int c = 0; c < 3; ++c) sp[c] += gp[c];
