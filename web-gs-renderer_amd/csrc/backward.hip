@@ -336,12 +336,13 @@ constexpr float SH_C3_0 = -0.5900435899266435f, SH_C3_1 = 2.890611442640554f,
                 SH_C3_6 = -0.5900435899266435f;
 
 // The raster's per-splat sums (moments of dL/dpower in slots 0-4, then
-// opacity and colour) in the 2-D gradient layout: d/dx = -(A Mx + B My),
-// d/dy = -(C My + B Mx), d/dA = -Mxx / 2, d/dB = -Mxy, d/dC = -Myy / 2, with
-// the splat's forward conic (A, B, C).
-__device__ __forceinline__ void moments_to_2d(float4 G0, float4 G1, float4 G2, float4 cn, float (&Gv)[9]) {
-    Gv[0] = -__builtin_fmaf(cn.x, G0.x, cn.y * G0.y);
-    Gv[1] = -__builtin_fmaf(cn.z, G0.y, cn.y * G0.x);
+// opacity and colour) in the 2-D gradient layout, except the position:
+// d/dA = -Mxx / 2, d/dB = -Mxy, d/dC = -Myy / 2; Gv[0], Gv[1] keep the
+// moments Mx, My, which view_adjoint<true> turns into d/dx = -(A Mx + B My),
+// d/dy = -(C My + B Mx) with the conic (A, B, C) it recomputes.
+__device__ __forceinline__ void sums_to_2d(float4 G0, float4 G1, float4 G2, float (&Gv)[9]) {
+    Gv[0] = G0.x;
+    Gv[1] = G0.y;
     Gv[2] = -0.5f * G0.z;
     Gv[3] = -G0.w;
     Gv[4] = -0.5f * G1.x;
@@ -611,12 +612,12 @@ __device__ __forceinline__ float opacity_of(const Par& par) {
 }
 
 // grad2d arrives with the raster's per-splat sums and leaves in the 2-D
-// gradient layout (moments_to_2d); grad receives the 59 parameter gradients.
+// gradient layout (the position's two terms formed with the conic that
+// view_adjoint recomputes); grad receives the 59 parameter gradients.
 // When the forward failed (frame_failed) every output is a quiet NaN, so a
 // caller that skips the error gsr_synchronize reports cannot take the result
 // for a gradient.
 __global__ __launch_bounds__(256) void k_preprocess_backward(const float* __restrict__ P, uint32_t n, CamK cam,
-                                                             const SplatRec* __restrict__ rec,
                                                              const FrameCounters* __restrict__ counters,
                                                              uint64_t capacity, float4* __restrict__ grad2d,
                                                              float* __restrict__ grad) {
@@ -629,10 +630,9 @@ __global__ __launch_bounds__(256) void k_preprocess_backward(const float* __rest
         for (int ch = 0; ch < (int)kParams; ++ch) grad[(size_t)ch * N + g] = q;
         return;
     }
-    // every load up front (sums, conic, the 59 parameters): one memory round
-    // trip per thread instead of one per dependent stage
+    // every load up front (sums, the 59 parameters): one memory round trip
+    // per thread instead of one per dependent stage
     const float4 G0 = grad2d[(size_t)g * 3], G1 = grad2d[(size_t)g * 3 + 1], G2 = grad2d[(size_t)g * 3 + 2];
-    const float4 cn = rec[g].co;
     float pv[kParams];
 #pragma unroll
     // streaming loads, as in preprocess (backward preprocess 113 -> 106 us)
@@ -640,18 +640,21 @@ __global__ __launch_bounds__(256) void k_preprocess_backward(const float* __rest
     const ParRegs par{pv};
     const bool any = (G0.x != 0.0f) | (G0.y != 0.0f) | (G0.z != 0.0f) | (G0.w != 0.0f) | (G1.x != 0.0f) |
                      (G1.y != 0.0f) | (G1.z != 0.0f) | (G1.w != 0.0f) | (G2.x != 0.0f);
+    // the 2-D gradient with the position's two moments left as they are
+    // (view_adjoint turns them into d/dx, d/dy: no read of the forward's
+    // record)
     float Gv[9];
-    if (any) {
-        moments_to_2d(G0, G1, G2, cn, Gv);
-        grad2d[(size_t)g * 3] = make_float4(Gv[0], Gv[1], Gv[2], Gv[3]);
-        grad2d[(size_t)g * 3 + 1] = make_float4(Gv[4], G1.y, G1.z, G1.w);
-    }
+    sums_to_2d(G0, G1, G2, Gv);
     SplatShape s;
     float gp[3], gsym[3][3], gch[3], basis[15], gs[7], gop = 0.0f;
     bool ok = any;
     if (ok) {
         splat_shape(par, s);
-        ok = view_adjoint(par, s, cam, Gv, gp, gsym, gch, basis);
+        ok = view_adjoint<true>(par, s, cam, Gv, gp, gsym, gch, basis);
+    }
+    if (any) {  // the 2-D gradient, in place of the sums
+        grad2d[(size_t)g * 3] = make_float4(ok ? Gv[0] : 0.0f, ok ? Gv[1] : 0.0f, Gv[2], Gv[3]);
+        grad2d[(size_t)g * 3 + 1] = make_float4(Gv[4], G1.y, G1.z, G1.w);
     }
     if (ok) {
         shape_adjoint(s, gsym, gs);
@@ -702,7 +705,6 @@ __device__ __forceinline__ const T* opaque(const T* p) {
 // receives every entry in the 2-D gradient layout.
 __global__ __launch_bounds__(256) void k_preprocess_backward_batch(const float* __restrict__ P, uint32_t n,
                                                                    const CamK* __restrict__ cams, uint32_t views,
-                                                                   const SplatRec* __restrict__ rec,
                                                                    const float4* __restrict__ grad2d,
                                                                    float4* __restrict__ grad2d_out,
                                                                    float* __restrict__ grad) {
@@ -730,7 +732,8 @@ __global__ __launch_bounds__(256) void k_preprocess_backward_batch(const float* 
         // the 2-D gradient with the position's two moments left as they are:
         // view_adjoint turns them into d/dx, d/dy with the conic it
         // recomputes (no read of the forward's record)
-        float Gv[9] = {G0.x, G0.y, -0.5f * G0.z, -G0.w, -0.5f * G1.x, G1.y, G1.z, G1.w, G2.x};
+        float Gv[9];
+        sums_to_2d(G0, G1, G2, Gv);
         if (!any) {
             if (grad2d_out) {
                 grad2d_out[e * 3] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -845,7 +848,7 @@ int launch_backward(Context* c, const float* params, uint32_t count, const CamK&
     GSR_HIP(c, hipGetLastError());
     mark(c, "backward preprocess");
     hipLaunchKernelGGL(k_preprocess_backward, dim3(ceil_div(count, 256)), dim3(256), 0, c->stream, params, count, cam,
-                       rec, counters, c->state_capacity, (float4*)g2, grad);
+                       counters, c->state_capacity, (float4*)g2, grad);
     GSR_HIP(c, hipGetLastError());
     mark_end(c, false);
     if (grad2d_out) GSR_HIP(c, hipMemcpyAsync(grad2d_out, g2, N * 48, hipMemcpyDeviceToDevice, c->stream));
@@ -895,7 +898,7 @@ int launch_backward_batch(Context* c, const float* params, uint32_t count, uint3
     GSR_HIP(c, hipGetLastError());
     mark(c, "backward preprocess");
     hipLaunchKernelGGL(k_preprocess_backward_batch, dim3(ceil_div(count, 256)), dim3(256), 0, c->stream, params, count,
-                       cams, views, rec, (const float4*)g2, (float4*)grad2d_out, grad);
+                       cams, views, (const float4*)g2, (float4*)grad2d_out, grad);
     GSR_HIP(c, hipGetLastError());
     mark_end(c, false);
     return GSR_OK;
