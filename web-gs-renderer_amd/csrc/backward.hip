@@ -426,6 +426,9 @@ __device__ __forceinline__ void splat_shape(const Par& par, SplatShape& s) {
 //   gch   dL/d(SH colour) per channel (zero where the forward clamped it);
 //   basis the 15 higher SH basis values of the view direction;
 // and the view direction's part of dL/dposition is added to gp.
+// POS_MOMENTS: Gv[0], Gv[1] arrive as the raster's moments sum g dx, sum g dy
+// and leave as d/dx, d/dy, formed with the conic of this function's own
+// covariance chain.
 template <bool POS_MOMENTS = false, typename Par, typename Shape>
 __device__ __forceinline__ bool view_adjoint(const Par& par, const Shape& s,
                                              const CamK& cam, float (&Gv)[9], float (&gp)[3],
