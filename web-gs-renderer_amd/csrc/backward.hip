@@ -338,7 +338,7 @@ constexpr float SH_C3_0 = -0.5900435899266435f, SH_C3_1 = 2.890611442640554f,
 // The raster's per-splat sums (moments of dL/dpower in slots 0-4, then
 // opacity and colour) in the 2-D gradient layout, except the position:
 // d/dA = -Mxx / 2, d/dB = -Mxy, d/dC = -Myy / 2; Gv[0], Gv[1] keep the
-// moments Mx, My, which view_adjoint<true> turns into d/dx = -(A Mx + B My),
+// moments Mx, My, which view_adjoint turns into d/dx = -(A Mx + B My),
 // d/dy = -(C My + B Mx) with the conic (A, B, C) it recomputes.
 __device__ __forceinline__ void sums_to_2d(float4 G0, float4 G1, float4 G2, float (&Gv)[9]) {
     Gv[0] = G0.x;
@@ -426,10 +426,10 @@ __device__ __forceinline__ void splat_shape(const Par& par, SplatShape& s) {
 //   gch   dL/d(SH colour) per channel (zero where the forward clamped it);
 //   basis the 15 higher SH basis values of the view direction;
 // and the view direction's part of dL/dposition is added to gp.
-// POS_MOMENTS: Gv[0], Gv[1] arrive as the raster's moments sum g dx, sum g dy
-// and leave as d/dx, d/dy, formed with the conic of this function's own
-// covariance chain.
-template <bool POS_MOMENTS = false, typename Par, typename Shape>
+// Gv[0], Gv[1] arrive as the raster's moments sum g dx, sum g dy and leave
+// as d/dx, d/dy, formed with the conic of this function's own covariance
+// chain.
+template <typename Par, typename Shape>
 __device__ __forceinline__ bool view_adjoint(const Par& par, const Shape& s,
                                              const CamK& cam, float (&Gv)[9], float (&gp)[3],
                                              float (&gsym)[3][3], float (&gch)[3], float (&basis)[15]) {
@@ -466,10 +466,8 @@ __device__ __forceinline__ bool view_adjoint(const Par& par, const Shape& s,
     const float b = Tm[0][0] * St1[0] + Tm[0][1] * St1[1] + Tm[0][2] * St1[2];
     const float c = Tm[1][0] * St1[0] + Tm[1][1] * St1[1] + Tm[1][2] * St1[2] + 0.3f;
     const float det = a * c - b * b;
-    if (POS_MOMENTS) {
-        // Gv[0], Gv[1] arrive as the moments sum g dx, sum g dy; the pixel
-        // position's gradient needs the conic, recomputed here from this
-        // chain (c, -b, a) / det instead of read from the forward's record
+    {   // the pixel position's gradient from its moments, with the conic
+        // (c, -b, a) / det of this chain (not read from the forward's record)
         const float A = c / det, B = -b / det, C = a / det;
         const float mx = Gv[0], my = Gv[1];
         Gv[0] = -__builtin_fmaf(A, mx, B * my);
@@ -653,7 +651,7 @@ __global__ __launch_bounds__(256) void k_preprocess_backward(const float* __rest
     bool ok = any;
     if (ok) {
         splat_shape(par, s);
-        ok = view_adjoint<true>(par, s, cam, Gv, gp, gsym, gch, basis);
+        ok = view_adjoint(par, s, cam, Gv, gp, gsym, gch, basis);
     }
     if (any) {  // the 2-D gradient, in place of the sums
         grad2d[(size_t)g * 3] = make_float4(ok ? Gv[0] : 0.0f, ok ? Gv[1] : 0.0f, Gv[2], Gv[3]);
@@ -759,7 +757,7 @@ __global__ __launch_bounds__(256) void k_preprocess_backward_batch(const float* 
             shaped = true;
         }
         float gp[3], gsym[3][3], gch[3], basis[15];
-        const bool ok = view_adjoint<true>(ParShLds{Pv, N, g, s_sh, col}, cv, cams[v], Gv, gp, gsym, gch, basis);
+        const bool ok = view_adjoint(ParShLds{Pv, N, g, s_sh, col}, cv, cams[v], Gv, gp, gsym, gch, basis);
         if (grad2d_out) {
             grad2d_out[e * 3] = make_float4(ok ? Gv[0] : 0.f, ok ? Gv[1] : 0.f, Gv[2], Gv[3]);
             grad2d_out[e * 3 + 1] = make_float4(Gv[4], Gv[5], Gv[6], Gv[7]);
