@@ -706,7 +706,7 @@ __device__ __forceinline__ const T* opaque(const T* p) {
 // receives every entry in the 2-D gradient layout.
 __global__ __launch_bounds__(256) void k_preprocess_backward_batch(const float* __restrict__ P, uint32_t n,
                                                                    const CamK* __restrict__ cams, uint32_t views,
-                                                                   const float4* __restrict__ grad2d,
+                                                                   float4* __restrict__ grad2d,
                                                                    float4* __restrict__ grad2d_out,
                                                                    float* __restrict__ grad) {
     // The SH coefficients are read per view; from HBM that was 192 B per
@@ -730,6 +730,11 @@ __global__ __launch_bounds__(256) void k_preprocess_backward_batch(const float* 
         const float4 G0 = grad2d[e * 3], G1 = grad2d[e * 3 + 1], G2 = grad2d[e * 3 + 2];
         const bool any = (G0.x != 0.0f) | (G0.y != 0.0f) | (G0.z != 0.0f) | (G0.w != 0.0f) | (G1.x != 0.0f) |
                          (G1.y != 0.0f) | (G1.z != 0.0f) | (G1.w != 0.0f) | (G2.x != 0.0f);
+        if (any) {  // read: clear it for the next backward's sums (the others are zero)
+            grad2d[e * 3] = make_float4(0.f, 0.f, 0.f, 0.f);
+            grad2d[e * 3 + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
+            grad2d[e * 3 + 2] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
         // the 2-D gradient with the position's two moments left as they are:
         // view_adjoint turns them into d/dx, d/dy with the conic it
         // recomputes (no read of the forward's record)
@@ -864,7 +869,10 @@ int launch_backward_batch(Context* c, const float* params, uint32_t count, uint3
     float* g2 = (float*)buffer(c, "b_grad2d", NV * 48);
     uint32_t* order = (uint32_t*)buffer(c, "b_tile_order", (size_t)total_tiles * 4);
     if (!g2 || !order) return GSR_ERR_ALLOC;
-    int rc = launch_fill_u32(c, (uint32_t*)g2, 0u, NV * 12);
+    // the previous batched backward cleared the sums it read, unless the
+    // buffer is new or of another shape
+    int rc = c->b_grad2d_clean == NV ? GSR_OK : launch_fill_u32(c, (uint32_t*)g2, 0u, NV * 12);
+    c->b_grad2d_clean = 0;
     if (rc) return rc;
     const uint32_t* vals = (const uint32_t*)buffer(c, sorted_index ? "b_vals1" : "b_vals0", 0);
     const uint2* ranges = (const uint2*)buffer(c, "b_ranges", 0);
@@ -899,8 +907,9 @@ int launch_backward_batch(Context* c, const float* params, uint32_t count, uint3
     GSR_HIP(c, hipGetLastError());
     mark(c, "backward preprocess");
     hipLaunchKernelGGL(k_preprocess_backward_batch, dim3(ceil_div(count, 256)), dim3(256), 0, c->stream, params, count,
-                       cams, views, (const float4*)g2, (float4*)grad2d_out, grad);
+                       cams, views, (float4*)g2, (float4*)grad2d_out, grad);
     GSR_HIP(c, hipGetLastError());
+    c->b_grad2d_clean = NV;
     mark_end(c, false);
     return GSR_OK;
 }
