@@ -159,6 +159,33 @@ def test_batch_backward_equals_sum_of_view_backwards(torch_cuda, ctx, tile):
     _check_view_sum(grad, want, mag)
 
 
+def test_repeated_batch_backward_gives_the_same_gradient(torch_cuda, ctx):
+    """The batched adjoint clears the 2-D gradient sums it reads, and the
+    next backward of the same shape skips its fill: two backward calls on one
+    saved forward, a third after a forward with other cameras and a fourth
+    back on the first, must each equal a fresh run's gradients (up to the
+    float atomics' order)."""
+    sc = scenes.clustered_scene(30_000, seed=13)
+    us = _views(sc.count, 3, seed=13)
+    us2 = _views(sc.count, 3, seed=14)
+    rng = np.random.default_rng(6)
+    dls = [rng.normal(size=(3, int(u[41]), int(u[40]))) for u in us]
+    params = to_dev(torch_cuda, sc.planar())
+    _say(f"repeated batch backward: n={sc.count} views=3")
+    first, first2d, _ = _batch_grads(torch_cuda, ctx, params, sc.count, us, dls, want_2d=True)
+    d = to_dev(torch_cuda, np.concatenate([np.ascontiguousarray(x, np.float32).ravel() for x in dls]))
+    grad = torch_cuda.empty(59 * sc.count, dtype=torch_cuda.float32, device="cuda")
+    ctx.backward_batch(params, sc.count, us, d, grad)   # same saved forward again
+    ctx.synchronize()
+    again = to_host(grad, np.float32).reshape(59, sc.count).astype(np.float64)
+    assert _norm_rel(again, first) < 1e-5
+    other, _, _ = _batch_grads(torch_cuda, ctx, params, sc.count, us2, dls)
+    assert _norm_rel(other, first) > 1e-2   # other cameras: another gradient
+    back, back2d, _ = _batch_grads(torch_cuda, ctx, params, sc.count, us, dls, want_2d=True)
+    assert _norm_rel(back, first) < 1e-5
+    assert _norm_rel(back2d, first2d) < 1e-5
+
+
 def test_batch_backward_state_checks(torch_cuda, ctx):
     sc = scenes.clustered_scene(10_000, seed=11)
     us = _views(sc.count, 2, seed=11)
