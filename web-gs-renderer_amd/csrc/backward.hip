@@ -612,22 +612,24 @@ __device__ __forceinline__ float opacity_of(const Par& par) {
     return 1.0f / (1.0f + gsr_exp(-par(CH_OPACITY)));
 }
 
-// grad2d arrives with the raster's per-splat sums and leaves in the 2-D
-// gradient layout (the position's two terms formed with the conic that
-// view_adjoint recomputes); grad receives the 59 parameter gradients.
-// When the forward failed (frame_failed) every output is a quiet NaN, so a
-// caller that skips the error gsr_synchronize reports cannot take the result
-// for a gradient.
+// grad2d arrives with the raster's per-splat sums and leaves cleared for the
+// next backward; out2d (nullable) receives the 2-D gradient (the position's
+// two terms formed with the conic that view_adjoint recomputes), grad the 59
+// parameter gradients.  When the forward failed (frame_failed) every output
+// is a quiet NaN, so a caller that skips the error gsr_synchronize reports
+// cannot take the result for a gradient.
 __global__ __launch_bounds__(256) void k_preprocess_backward(const float* __restrict__ P, uint32_t n, CamK cam,
                                                              const FrameCounters* __restrict__ counters,
                                                              uint64_t capacity, float4* __restrict__ grad2d,
+                                                             float4* __restrict__ out2d,
                                                              float* __restrict__ grad) {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
     const size_t N = n;
     if (frame_failed(counters, capacity)) {
         const float q = __builtin_nanf("");
-        for (int k = 0; k < 3; ++k) grad2d[(size_t)g * 3 + k] = make_float4(q, q, q, q);
+        if (out2d)
+            for (int k = 0; k < 3; ++k) out2d[(size_t)g * 3 + k] = make_float4(q, q, q, q);
         for (int ch = 0; ch < (int)kParams; ++ch) grad[(size_t)ch * N + g] = q;
         return;
     }
@@ -653,9 +655,16 @@ __global__ __launch_bounds__(256) void k_preprocess_backward(const float* __rest
         splat_shape(par, s);
         ok = view_adjoint(par, s, cam, Gv, gp, gsym, gch, basis);
     }
-    if (any) {  // the 2-D gradient, in place of the sums
-        grad2d[(size_t)g * 3] = make_float4(ok ? Gv[0] : 0.0f, ok ? Gv[1] : 0.0f, Gv[2], Gv[3]);
-        grad2d[(size_t)g * 3 + 1] = make_float4(Gv[4], G1.y, G1.z, G1.w);
+    if (any) {  // read: clear it for the next backward's sums (the others are zero)
+        grad2d[(size_t)g * 3] = make_float4(0.f, 0.f, 0.f, 0.f);
+        grad2d[(size_t)g * 3 + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
+        grad2d[(size_t)g * 3 + 2] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    if (out2d) {  // zeros (not -0) for a splat without sums
+        out2d[(size_t)g * 3] = any ? make_float4(ok ? Gv[0] : 0.0f, ok ? Gv[1] : 0.0f, Gv[2], Gv[3])
+                                   : make_float4(0.f, 0.f, 0.f, 0.f);
+        out2d[(size_t)g * 3 + 1] = any ? make_float4(Gv[4], Gv[5], Gv[6], Gv[7]) : make_float4(0.f, 0.f, 0.f, 0.f);
+        out2d[(size_t)g * 3 + 2] = make_float4(any ? Gv[8] : 0.0f, 0.0f, 0.0f, 0.0f);
     }
     if (ok) {
         shape_adjoint(s, gsym, gs);
@@ -821,7 +830,10 @@ int launch_backward(Context* c, const float* params, uint32_t count, const CamK&
     const size_t N = count;
     float* g2 = (float*)buffer(c, "grad2d", N * 48);
     if (!g2) return GSR_ERR_ALLOC;
-    int rc = launch_fill_u32(c, (uint32_t*)g2, 0u, N * 12);
+    // the previous backward cleared the sums it read, unless the buffer is
+    // new or of another size
+    int rc = c->grad2d_clean == N ? GSR_OK : launch_fill_u32(c, (uint32_t*)g2, 0u, N * 12);
+    c->grad2d_clean = 0;
     if (rc) return rc;
     const uint32_t* vals = (const uint32_t*)buffer(c, c->state_sorted_index ? "vals1" : "vals0", 0);
     const uint2* ranges = (const uint2*)buffer(c, "ranges", 0);
@@ -854,10 +866,10 @@ int launch_backward(Context* c, const float* params, uint32_t count, const CamK&
     GSR_HIP(c, hipGetLastError());
     mark(c, "backward preprocess");
     hipLaunchKernelGGL(k_preprocess_backward, dim3(ceil_div(count, 256)), dim3(256), 0, c->stream, params, count, cam,
-                       counters, c->state_capacity, (float4*)g2, grad);
+                       counters, c->state_capacity, (float4*)g2, (float4*)grad2d_out, grad);
     GSR_HIP(c, hipGetLastError());
+    c->grad2d_clean = N;
     mark_end(c, false);
-    if (grad2d_out) GSR_HIP(c, hipMemcpyAsync(grad2d_out, g2, N * 48, hipMemcpyDeviceToDevice, c->stream));
     return GSR_OK;
 }
 

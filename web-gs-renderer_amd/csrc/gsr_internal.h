@@ -187,6 +187,7 @@ struct Context {
     // entries of b_grad2d that the last batched backward left zero (it
     // clears the sums it has read; 0: unknown, fill before use)
     size_t b_grad2d_clean = 0;
+    size_t grad2d_clean = 0;     // the same for the plain backward's sums
     uint32_t last_frame_w = 0, last_frame_h = 0;  // size of the frame in "image"
     // prefix of the scan/sort scratch buffer names ("" plain frames, "b_" batches)
     const char* scratch_prefix = "";
