@@ -213,15 +213,17 @@ __global__ __launch_bounds__(1024) void k_tile_order(const uint2* __restrict__ r
     // counter of tile t for this lane: heaviest class first, then the copy
     auto slot_of = [&](uint2 r) -> uint32_t { return (127 - cls(r.y - r.x)) * kOrderCopies + (lane % kOrderCopies); };
     // Each pass loads kOrderItems ranges per thread before using any (one
-    // memory round trip per 8192 tiles).
+    // memory round trip per 8192 tiles); with at most 8192 tiles the second
+    // pass reuses the first one's ranges.
+    uint2 r[kOrderItems];
     for (uint32_t c0 = t0; c0 < t1; c0 += 1024 * kOrderItems) {
-        uint2 r[kOrderItems];
 #pragma unroll
         for (uint32_t i = 0; i < kOrderItems; ++i) r[i] = ranges[min(c0 + i * 1024 + tid, t1 - 1)];  // in range
 #pragma unroll
         for (uint32_t i = 0; i < kOrderItems; ++i)
             if (c0 + i * 1024 + tid < t1) atomicAdd(&s_cnt[slot_of(r[i])], 1u);
     }
+    const bool one_chunk = t1 - t0 <= 1024 * kOrderItems;
     __syncthreads();
     {   // exclusive scan of the counters in index order: 4 per thread
         const uint32_t c[4] = {s_cnt[4 * tid], s_cnt[4 * tid + 1], s_cnt[4 * tid + 2], s_cnt[4 * tid + 3]};
@@ -238,9 +240,9 @@ __global__ __launch_bounds__(1024) void k_tile_order(const uint2* __restrict__ r
     }
     __syncthreads();
     for (uint32_t c0 = t0; c0 < t1; c0 += 1024 * kOrderItems) {
-        uint2 r[kOrderItems];
+        if (!one_chunk)
 #pragma unroll
-        for (uint32_t i = 0; i < kOrderItems; ++i) r[i] = ranges[min(c0 + i * 1024 + tid, t1 - 1)];  // in range
+            for (uint32_t i = 0; i < kOrderItems; ++i) r[i] = ranges[min(c0 + i * 1024 + tid, t1 - 1)];  // in range
 #pragma unroll
         for (uint32_t i = 0; i < kOrderItems; ++i) {
             const uint32_t t = c0 + i * 1024 + tid;
