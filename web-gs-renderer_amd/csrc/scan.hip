@@ -72,46 +72,56 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(const uint32_t* in, uint3
         if (w < (int)wave) wave_prefix += t;
         block_total += t;
     }
-    if (tid == 0) {
-        uint64_t excl = 0;
-        if (tile == 0) {
+    if (tile == 0) {
+        if (tid == 0) {
             st_release_u64(&status[0], kFlagInc | block_total);
-        } else {
-            st_release_u64(&status[tile], kFlagAgg | block_total);
-            // batched look-back (see sort.hip): kLookBack words per round trip
-            int64_t j = (int64_t)tile - 1;
-            uint32_t spins = 0;
-            bool found = false;
-            while (!found && j >= 0) {
-                uint64_t w[kLookBack];
+            s_prefix = 0;
+        }
+    } else if (wave == 0) {
+        if (lane == 0) st_release_u64(&status[tile], kFlagAgg | block_total);
+        // Look-back by the whole first wave: lane l polls predecessor
+        // j - l, 64 per round trip.  The exclusive prefix sums the words up
+        // to the nearest inclusive one; a round is used only once every word
+        // it needs is published (the others are polled again).
+        uint64_t excl = 0;
+        int64_t j = (int64_t)tile - 1;
+        uint32_t spins = 0;
+        bool done = false;
+        while (!done) {
+            const int64_t p = j - (int64_t)lane;
+            uint64_t s = p >= 0 ? ld_acquire_u64(&status[p]) : kFlagInc;  // before tile 0: an inclusive 0
+            for (;;) {
+                const uint64_t inc = ballot((s >> 62) == 2u);
+                const uint32_t first = inc ? (uint32_t)__builtin_ctzll(inc) : 63u;
+                const uint64_t need = (2ull << first) - 1ull;  // lanes 0 .. first (all 64 for 63)
+                if ((ballot((s >> 62) != 0u) & need) == need) {
+                    uint64_t v = lane <= first ? (s & kValMask) : 0ull;
 #pragma unroll
-                for (int d = 0; d < kLookBack; ++d) w[d] = (j - d >= 0) ? ld_acquire_u64(&status[j - d]) : kFlagInc;
-#pragma unroll
-                for (int d = 0; d < kLookBack; ++d) {
-                    if (found || j - d < 0) continue;
-                    uint64_t s = w[d];
-                    while ((s >> 62) == 0) {
-                        if (++spins > kSpinLimit) {
-                            atomicOr(&counters->error, 1u);
-                            s = kFlagInc;
-                            break;
-                        }
-                        __builtin_amdgcn_s_sleep(1);
-                        s = ld_acquire_u64(&status[j - d]);
-                    }
-                    excl += s & kValMask;
-                    if (s & kFlagInc) found = true;
+                    for (int m = 32; m >= 1; m >>= 1) v += (uint64_t)__shfl_xor((long long)v, m, 64);
+                    excl += v;
+                    done = inc != 0;
+                    break;
                 }
-                j -= kLookBack;
+                if (++spins > kSpinLimit) {
+                    if (lane == 0) atomicOr(&counters->error, 1u);
+                    done = true;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+                if ((s >> 62) == 0u) s = ld_acquire_u64(&status[p]);  // p >= 0: the virtual word is published
             }
+            j -= 64;
+        }
+        if (lane == 0) {
             st_release_u64(&status[tile], kFlagInc | ((excl + block_total) & kValMask));
+            s_prefix = excl;
         }
-        s_prefix = excl;
-        if (tile == num_tiles - 1) {
-            const uint64_t total = excl + block_total;
-            counters->instances = total;
-            counters->n = (uint32_t)(total < capacity ? total : capacity);
-        }
+    }
+    if (tid == 0 && tile == num_tiles - 1) {
+        // (the last tile's s_prefix is written above by this same thread)
+        const uint64_t total = s_prefix + block_total;
+        counters->instances = total;
+        counters->n = (uint32_t)(total < capacity ? total : capacity);
     }
     __syncthreads();
     uint64_t run = s_prefix + wave_prefix + (incl - sum);
