@@ -46,11 +46,16 @@ constexpr uint32_t kAgg = 1u << 30, kInc = 2u << 30;
 constexpr int kLookBack = 16;   // predecessor words fetched per look-back step (all digits of a thread)
 constexpr int kHistItems = 8;   // keys per thread per histogram block tile
 // Copies of the global digit histograms that the histogram blocks add into
-// (block b into copy b % kHistCopies; onesweep sums them): one copy took the
-// flush of some 400K atomics into 32 cache lines (1080p frame), which
-// serialise per line; 8 copies: frame sort 0.1148 -> 0.1122 ms.  Scratch the
-// caller provides (the no-allocation sort) keeps one copy.
+// (block b into copy b % copies; onesweep sums them), so that the flush of
+// some 400K atomics (1080p frame) does not serialise on one set of cache
+// lines.  Measured in one box session on the 1080p frame's 32-bit keys: 1,
+// 2, 4, 8 and 16 copies gave a sort of 0.1143, 0.1129, 0.1143, 0.1191 and
+// 0.1267 ms; on the 64-view batch's 64-bit keys 1 and 8 copies were equal
+// and 2 or 4 1.5% slower.  kHistCopies is the most any sort uses; scratch
+// the caller provides (the no-allocation sort) keeps one copy.
 constexpr uint32_t kHistCopies = 8;
+template <typename K>
+constexpr uint32_t hist_copies() { return sizeof(K) == 4 ? 2u : kHistCopies; }
 // onesweep keys per thread at most for 64-bit keys in 1024-thread blocks
 // (8 in smaller blocks, whose several blocks per CU need fewer registers)
 constexpr int kMaxItems64 = 12;
@@ -542,7 +547,7 @@ int sort_impl(Context* c, K* k0, uint32_t* v0, K* k1, uint32_t* v1, const uint32
     if (passes == 0) return GSR_OK;
     if (passes > (uint32_t)kMaxPasses) return fail(c, GSR_ERR_ARGUMENT, "too many radix passes (%u)", passes);
     const uint32_t slots = (uint32_t)c->cu_count * (1024 / THREADS);
-    const uint32_t copies = mem ? 1u : kHistCopies;
+    const uint32_t copies = mem ? 1u : hist_copies<K>();
     const SortWords words = sort_words_of<K, THREADS, RB>(c, capacity, begin_bit, end_bit, copies);
     const size_t hist_words = words.hist;
     const uint32_t max_tiles = (uint32_t)(words.status / (2 * R));
