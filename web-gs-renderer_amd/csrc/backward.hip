@@ -17,8 +17,10 @@
 //   the 2-D gradient (pixel position, conic, opacity, colour) and chains it
 //   back to the 59 parameters.
 //
-// 2-D gradient layout per splat (12 floats): d/dx, d/dy of the pixel
-// position; d/dA, d/dB, d/dC of the conic; d/d opacity; d/d(r, g, b); pad.
+// 2-D gradient layout per splat as returned in grad2d_out (12 floats): d/dx,
+// d/dy of the pixel position; d/dA, d/dB, d/dC of the conic; d/d opacity;
+// d/d(r, g, b); pad.  The raster's sums that it is made from take 9 floats
+// per entry (no pad).
 #include <cstddef>
 #include <cstring>
 
@@ -65,6 +67,11 @@ static_assert(sizeof(BwdWaveLds) == 5632, "backward LDS per wave");
 // 16-lane group holds after backward_flush's reduce-scatter (4 bits per
 // lane, 0xF: a duplicate, not added).
 constexpr uint64_t kFlushTotalOfLane = 0xFFF8F765FF43F210ull;
+
+// Floats per 2-D gradient-sum entry (one per splat, or per (view, splat) in
+// a batch): the five position/conic moments, the opacity term and the
+// three colour terms, packed without padding.
+constexpr uint32_t kSumFloats = 9;
 
 // Sums the collected splats' terms per splat and adds them to the splats'
 // 2-D gradient sums.  Lane l works on collected splat c = l / 16, block
@@ -145,7 +152,7 @@ __device__ __forceinline__ void backward_flush(BwdWaveLds& L, uint32_t n, uint64
     __builtin_amdgcn_wave_barrier();
     const uint32_t slot = (uint32_t)(kFlushTotalOfLane >> (4 * (lane & 15))) & 0xFu;
     if (c < n && gid < splats && slot != 0xFu) {
-        float* dst = grad2d + (size_t)gid * 12;
+        float* dst = grad2d + (size_t)gid * kSumFloats;
         unsafeAtomicAdd(dst + slot, mine);
     }
 }
@@ -340,16 +347,35 @@ constexpr float SH_C3_0 = -0.5900435899266435f, SH_C3_1 = 2.890611442640554f,
 // d/dA = -Mxx / 2, d/dB = -Mxy, d/dC = -Myy / 2; Gv[0], Gv[1] keep the
 // moments Mx, My, which view_adjoint turns into d/dx = -(A Mx + B My),
 // d/dy = -(C My + B Mx) with the conic (A, B, C) it recomputes.
-__device__ __forceinline__ void sums_to_2d(float4 G0, float4 G1, float4 G2, float (&Gv)[9]) {
-    Gv[0] = G0.x;
-    Gv[1] = G0.y;
-    Gv[2] = -0.5f * G0.z;
-    Gv[3] = -G0.w;
-    Gv[4] = -0.5f * G1.x;
-    Gv[5] = G1.y;
-    Gv[6] = G1.z;
-    Gv[7] = G1.w;
-    Gv[8] = G2.x;
+__device__ __forceinline__ void sums_to_2d(const float (&S)[kSumFloats], float (&Gv)[9]) {
+    Gv[0] = S[0];
+    Gv[1] = S[1];
+    Gv[2] = -0.5f * S[2];
+    Gv[3] = -S[3];
+    Gv[4] = -0.5f * S[4];
+    Gv[5] = S[5];
+    Gv[6] = S[6];
+    Gv[7] = S[7];
+    Gv[8] = S[8];
+}
+
+// One entry's sums: 36 bytes at a 4-byte alignment, which the loads and
+// stores of consecutive threads cover without gaps.  Returns whether any
+// of them is nonzero (sums start at +0 and atomic adds never make -0, so
+// a zero entry is all +0 and needs no clearing).
+__device__ __forceinline__ bool load_sums(const float* __restrict__ e, float (&S)[kSumFloats]) {
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < (int)kSumFloats; ++k) {
+        S[k] = e[k];
+        any |= S[k] != 0.0f;
+    }
+    return any;
+}
+
+__device__ __forceinline__ void clear_sums(float* __restrict__ e) {
+#pragma unroll
+    for (int k = 0; k < (int)kSumFloats; ++k) e[k] = 0.0f;
 }
 
 // View-independent part of one splat for the preprocess adjoint (the
@@ -620,7 +646,7 @@ __device__ __forceinline__ float opacity_of(const Par& par) {
 // cannot take the result for a gradient.
 __global__ __launch_bounds__(256) void k_preprocess_backward(const float* __restrict__ P, uint32_t n, CamK cam,
                                                              const FrameCounters* __restrict__ counters,
-                                                             uint64_t capacity, float4* __restrict__ grad2d,
+                                                             uint64_t capacity, float* __restrict__ grad2d,
                                                              float4* __restrict__ out2d,
                                                              float* __restrict__ grad) {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
@@ -635,19 +661,18 @@ __global__ __launch_bounds__(256) void k_preprocess_backward(const float* __rest
     }
     // every load up front (sums, the 59 parameters): one memory round trip
     // per thread instead of one per dependent stage
-    const float4 G0 = grad2d[(size_t)g * 3], G1 = grad2d[(size_t)g * 3 + 1], G2 = grad2d[(size_t)g * 3 + 2];
+    float S[kSumFloats];
+    const bool any = load_sums(grad2d + (size_t)g * kSumFloats, S);
     float pv[kParams];
 #pragma unroll
     // streaming loads, as in preprocess (backward preprocess 113 -> 106 us)
     for (int ch = 0; ch < (int)kParams; ++ch) pv[ch] = __builtin_nontemporal_load(P + (size_t)ch * N + g);
     const ParRegs par{pv};
-    const bool any = (G0.x != 0.0f) | (G0.y != 0.0f) | (G0.z != 0.0f) | (G0.w != 0.0f) | (G1.x != 0.0f) |
-                     (G1.y != 0.0f) | (G1.z != 0.0f) | (G1.w != 0.0f) | (G2.x != 0.0f);
     // the 2-D gradient with the position's two moments left as they are
     // (view_adjoint turns them into d/dx, d/dy: no read of the forward's
     // record)
     float Gv[9];
-    sums_to_2d(G0, G1, G2, Gv);
+    sums_to_2d(S, Gv);
     SplatShape s;
     float gp[3], gsym[3][3], gch[3], basis[15], gs[7], gop = 0.0f;
     bool ok = any;
@@ -655,11 +680,7 @@ __global__ __launch_bounds__(256) void k_preprocess_backward(const float* __rest
         splat_shape(par, s);
         ok = view_adjoint(par, s, cam, Gv, gp, gsym, gch, basis);
     }
-    if (any) {  // read: clear it for the next backward's sums (the others are zero)
-        grad2d[(size_t)g * 3] = make_float4(0.f, 0.f, 0.f, 0.f);
-        grad2d[(size_t)g * 3 + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
-        grad2d[(size_t)g * 3 + 2] = make_float4(0.f, 0.f, 0.f, 0.f);
-    }
+    if (any) clear_sums(grad2d + (size_t)g * kSumFloats);  // read: cleared for the next backward's sums
     if (out2d) {  // zeros (not -0) for a splat without sums
         out2d[(size_t)g * 3] = any ? make_float4(ok ? Gv[0] : 0.0f, ok ? Gv[1] : 0.0f, Gv[2], Gv[3])
                                    : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -713,9 +734,10 @@ __device__ __forceinline__ const T* opaque(const T* p) {
 // values) before the scale / rotation chain, dL/d opacity before the sigmoid
 // slope, dL/d colour before the SH-DC constant.  grad2d_out (nullable)
 // receives every entry in the 2-D gradient layout.
-__global__ __launch_bounds__(256) void k_preprocess_backward_batch(const float* __restrict__ P, uint32_t n,
+// At most 168 VGPRs: 3 waves per SIMD (the 48 KB SH table allows 3 workgroups per CU)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_preprocess_backward_batch(const float* __restrict__ P, uint32_t n,
                                                                    const CamK* __restrict__ cams, uint32_t views,
-                                                                   float4* __restrict__ grad2d,
+                                                                   float* __restrict__ grad2d,
                                                                    float4* __restrict__ grad2d_out,
                                                                    float* __restrict__ grad) {
     // The SH coefficients are read per view; from HBM that was 192 B per
@@ -736,19 +758,14 @@ __global__ __launch_bounds__(256) void k_preprocess_backward_batch(const float* 
     float sp[3] = {0.f, 0.f, 0.f}, ss[6] = {}, sop = 0.0f, sch[3] = {0.f, 0.f, 0.f}, srest[45] = {};
     for (uint32_t v = 0; v < views; ++v) {
         const size_t e = (size_t)v * N + g;
-        const float4 G0 = grad2d[e * 3], G1 = grad2d[e * 3 + 1], G2 = grad2d[e * 3 + 2];
-        const bool any = (G0.x != 0.0f) | (G0.y != 0.0f) | (G0.z != 0.0f) | (G0.w != 0.0f) | (G1.x != 0.0f) |
-                         (G1.y != 0.0f) | (G1.z != 0.0f) | (G1.w != 0.0f) | (G2.x != 0.0f);
-        if (any) {  // read: clear it for the next backward's sums (the others are zero)
-            grad2d[e * 3] = make_float4(0.f, 0.f, 0.f, 0.f);
-            grad2d[e * 3 + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
-            grad2d[e * 3 + 2] = make_float4(0.f, 0.f, 0.f, 0.f);
-        }
+        float S[kSumFloats];
+        const bool any = load_sums(grad2d + e * kSumFloats, S);
+        if (any) clear_sums(grad2d + e * kSumFloats);  // read: cleared for the next backward's sums
         // the 2-D gradient with the position's two moments left as they are:
         // view_adjoint turns them into d/dx, d/dy with the conic it
         // recomputes (no read of the forward's record)
         float Gv[9];
-        sums_to_2d(G0, G1, G2, Gv);
+        sums_to_2d(S, Gv);
         if (!any) {
             if (grad2d_out) {
                 grad2d_out[e * 3] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -828,11 +845,11 @@ int launch_backward(Context* c, const float* params, uint32_t count, const CamK&
     const uint32_t W = c->state_w, H = c->state_h, tile = c->state_tile;
     const uint32_t tx = ceil_div(W, tile), ty = ceil_div(H, tile);
     const size_t N = count;
-    float* g2 = (float*)buffer(c, "grad2d", N * 48);
+    float* g2 = (float*)buffer(c, "grad2d", N * kSumFloats * 4);
     if (!g2) return GSR_ERR_ALLOC;
     // the previous backward cleared the sums it read, unless the buffer is
     // new or of another size
-    int rc = c->grad2d_clean == N ? GSR_OK : launch_fill_u32(c, (uint32_t*)g2, 0u, N * 12);
+    int rc = c->grad2d_clean == N ? GSR_OK : launch_fill_u32(c, (uint32_t*)g2, 0u, N * kSumFloats);
     c->grad2d_clean = 0;
     if (rc) return rc;
     const uint32_t* vals = (const uint32_t*)buffer(c, c->state_sorted_index ? "vals1" : "vals0", 0);
@@ -866,7 +883,7 @@ int launch_backward(Context* c, const float* params, uint32_t count, const CamK&
     GSR_HIP(c, hipGetLastError());
     mark(c, "backward preprocess");
     hipLaunchKernelGGL(k_preprocess_backward, dim3(ceil_div(count, 256)), dim3(256), 0, c->stream, params, count, cam,
-                       counters, c->state_capacity, (float4*)g2, (float4*)grad2d_out, grad);
+                       counters, c->state_capacity, g2, (float4*)grad2d_out, grad);
     GSR_HIP(c, hipGetLastError());
     c->grad2d_clean = N;
     mark_end(c, false);
@@ -878,12 +895,12 @@ int launch_backward_batch(Context* c, const float* params, uint32_t count, uint3
                           int sorted_index, const float* dLdC, float* grad, float* grad2d_out) {
     if (count == 0) return GSR_OK;  // no splats, no gradients
     const size_t NV = (size_t)count * views;
-    float* g2 = (float*)buffer(c, "b_grad2d", NV * 48);
+    float* g2 = (float*)buffer(c, "b_grad2d", NV * kSumFloats * 4);
     uint32_t* order = (uint32_t*)buffer(c, "b_tile_order", (size_t)total_tiles * 4);
     if (!g2 || !order) return GSR_ERR_ALLOC;
     // the previous batched backward cleared the sums it read, unless the
     // buffer is new or of another shape
-    int rc = c->b_grad2d_clean == NV ? GSR_OK : launch_fill_u32(c, (uint32_t*)g2, 0u, NV * 12);
+    int rc = c->b_grad2d_clean == NV ? GSR_OK : launch_fill_u32(c, (uint32_t*)g2, 0u, NV * kSumFloats);
     c->b_grad2d_clean = 0;
     if (rc) return rc;
     const uint32_t* vals = (const uint32_t*)buffer(c, sorted_index ? "b_vals1" : "b_vals0", 0);
@@ -919,7 +936,7 @@ int launch_backward_batch(Context* c, const float* params, uint32_t count, uint3
     GSR_HIP(c, hipGetLastError());
     mark(c, "backward preprocess");
     hipLaunchKernelGGL(k_preprocess_backward_batch, dim3(ceil_div(count, 256)), dim3(256), 0, c->stream, params, count,
-                       cams, views, (float4*)g2, (float4*)grad2d_out, grad);
+                       cams, views, g2, (float4*)grad2d_out, grad);
     GSR_HIP(c, hipGetLastError());
     c->b_grad2d_clean = NV;
     mark_end(c, false);
