@@ -38,7 +38,7 @@ RASTER_OPS_PER_EVALUATION = 15
 # TCC_EA0_WRREQ/_64B) summarised in profiles/forward_1m_1080p_pmc_traffic.md,
 # which FETCH_SIZE / WRITE_SIZE calibrated against a plain copy confirm; a
 # counter run cannot be part of this process, so the figure is the profile's.
-RASTER_TRAFFIC_BYTES = 460823671
+RASTER_TRAFFIC_BYTES = 460405979
 RASTER_TRAFFIC_SOURCE = ("profiles/forward_1m_1080p_pmc_traffic.md (rocprofv3 read/write request counters by size; "
                          "FETCH_SIZE + WRITE_SIZE calibrated against a copy agree)")
 
